@@ -1,0 +1,287 @@
+/*
+ * pfft.h -- public C ABI of the MI355X-native PFFT transform pipeline
+ * (library: pfft_amd/lib/libpfft.so).
+ *
+ * This header is the drop-in boundary for code written against the
+ * reference's api/pfft.h on a 1x1 process mesh (one process, one GPU).
+ * Names, argument meaning, flag values and error behaviour follow the
+ * reference; each group below cites the reference declaration it
+ * replaces.  Arrays passed to plans and execute calls may be device
+ * memory (pfft_malloc/alloc_* return hipMalloc'ed memory) or host memory
+ * (staged through device buffers; the PCIe-inclusive path).
+ *
+ * Two precisions are exported, as in the reference (api/pfft.h:562-564):
+ *   pfft_   double        complex = double[2]
+ *   pfftf_  float         complex = float[2]
+ * The long-double pfftl_ family has no GPU arithmetic and is not exported.
+ */
+#ifndef PFFT_H
+#define PFFT_H 1
+
+#include <stdarg.h>
+#include <stddef.h>
+#include <stdio.h>
+
+/* MPI communicator handles.  The library never calls MPI: on a 1x1 mesh
+ * every communicator has one rank.  If the caller included an MPICH-ABI
+ * <mpi.h> first (MPI_VERSION defined), its MPI_Comm (an int handle) is
+ * used; otherwise a minimal single-rank definition with the same ABI is
+ * provided here. */
+#ifndef MPI_VERSION
+typedef int MPI_Comm;
+#  define MPI_COMM_WORLD ((MPI_Comm)0x44000000)
+#  define MPI_COMM_SELF  ((MPI_Comm)0x44000001)
+#  define MPI_COMM_NULL  ((MPI_Comm)0x04000000)
+#  define PFFT_SINGLE_RANK_MPI 1
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FFTW constants restated (FFTW 3.3 fftw3.h; reference api/pfft.h:44-59) */
+#define PFFT_FORWARD   (-1)
+#define PFFT_BACKWARD  (+1)
+typedef enum {
+  PFFT_R2HC = 0, PFFT_HC2R = 1, PFFT_DHT = 2,
+  PFFT_REDFT00 = 3, PFFT_REDFT01 = 4, PFFT_REDFT10 = 5, PFFT_REDFT11 = 6,
+  PFFT_RODFT00 = 7, PFFT_RODFT01 = 8, PFFT_RODFT10 = 9, PFFT_RODFT11 = 10
+} pfft_r2r_kind;
+typedef pfft_r2r_kind pfftf_r2r_kind;
+
+typedef double pfft_complex[2];
+typedef float pfftf_complex[2];
+
+/* Planner flags: same bit values as the reference (api/pfft.h:528-545). */
+#define PFFT_TRANSPOSED_NONE      (0U)
+#define PFFT_TRANSPOSED_IN        (1U<< 0)
+#define PFFT_TRANSPOSED_OUT       (1U<< 1)
+#define PFFT_SHIFTED_NONE         (0U)
+#define PFFT_SHIFTED_IN           (1U<< 2)
+#define PFFT_SHIFTED_OUT          (1U<< 3)
+#define PFFT_MEASURE              (0U)
+#define PFFT_ESTIMATE             (1U<< 4)
+#define PFFT_PATIENT              (1U<< 5)
+#define PFFT_EXHAUSTIVE           (1U<< 6)
+#define PFFT_NO_TUNE              (0U)
+#define PFFT_TUNE                 (1U<< 7)
+#define PFFT_PRESERVE_INPUT       (1U<< 8)
+#define PFFT_DESTROY_INPUT        (1U<< 9)
+#define PFFT_BUFFERED_INPLACE     (1U<<10)
+#define PFFT_PADDED_R2C           (1U<<11)
+#define PFFT_PADDED_C2R           (PFFT_PADDED_R2C)
+
+/* Extension bits (>= 1<<12; unused by the reference): arithmetic form of
+ * the pfftf_ (complex64) plans.  Default: plain FP32 Stockham. */
+#define PFFT_AMD_MFMA_F32         (1U<<12)  /* radix-16 stages as FP32 MFMA products */
+#define PFFT_AMD_MFMA_BF16X2      (1U<<13)  /* split hi+lo BF16 MFMA, FP32 accumulate */
+
+#define PFFT_DEFAULT_BLOCK        (0)       /* FFTW_MPI_DEFAULT_BLOCK */
+#define PFFT_DEFAULT_BLOCKS       NULL
+#define PFFT_NO_GCELLS            NULL
+
+/*
+ * One declaration list for both precisions.
+ *   P: name prefix   R: real type   C: complex type   INT: ptrdiff_t
+ */
+#define PFFT_AMD_DECLARE_API(P, R, C, INT)                                          \
+  typedef struct P##plan_s *P##plan;                                                \
+  /* setup: api/pfft.h:77-78 (init, cleanup), 79-80 (threads) */                   \
+  void P##init(void);                                                               \
+  void P##cleanup(void);                                                            \
+  void P##plan_with_nthreads(int nthreads);                                         \
+  int  P##get_nthreads(void);                                                       \
+  /* memory: api/pfft.h:82-85 */                                                    \
+  void *P##malloc(size_t n);                                                        \
+  R    *P##alloc_real(size_t n);                                                    \
+  C    *P##alloc_complex(size_t n);                                                 \
+  void  P##free(void *p);                                                           \
+  /* execution: api/pfft.h:87-92 */                                                 \
+  void P##execute(const P##plan plan);                                              \
+  void P##execute_dft(const P##plan plan, C *in, C *out);                           \
+  void P##execute_dft_r2c(const P##plan plan, R *in, C *out);                       \
+  void P##execute_dft_c2r(const P##plan plan, C *in, R *out);                       \
+  void P##execute_r2r(const P##plan plan, R *in, R *out);                           \
+  void P##destroy_plan(P##plan plan);                                               \
+  /* test-data tools: api/pfft.h:94-149 */                                          \
+  void P##init_input_complex_3d(const INT *n, const INT *local_n,                   \
+      const INT *local_n_start, C *data);                                           \
+  void P##init_input_complex(int rnk_n, const INT *n, const INT *local_n,           \
+      const INT *local_start, C *data);                                             \
+  void P##init_input_complex_hermitian_3d(const INT *n, const INT *local_n,         \
+      const INT *local_n_start, C *data);                                           \
+  void P##init_input_complex_hermitian(int rnk_n, const INT *n,                     \
+      const INT *local_n, const INT *local_start, C *data);                         \
+  void P##init_input_real_3d(const INT *n, const INT *local_n,                      \
+      const INT *local_n_start, R *data);                                           \
+  void P##init_input_real(int rnk_n, const INT *n, const INT *local_n,              \
+      const INT *local_start, R *data);                                             \
+  void P##clear_input_complex_3d(const INT *n, const INT *local_n,                  \
+      const INT *local_n_start, C *data);                                           \
+  void P##clear_input_complex(int rnk_n, const INT *n, const INT *local_n,          \
+      const INT *local_start, C *data);                                             \
+  void P##clear_input_complex_hermitian_3d(const INT *n, const INT *local_n,        \
+      const INT *local_n_start, C *data);                                           \
+  void P##clear_input_complex_hermitian(int rnk_n, const INT *n,                    \
+      const INT *local_n, const INT *local_start, C *data);                         \
+  void P##clear_input_real_3d(const INT *n, const INT *local_n,                     \
+      const INT *local_n_start, R *data);                                           \
+  void P##clear_input_real(int rnk_n, const INT *n, const INT *local_n,             \
+      const INT *local_start, R *data);                                             \
+  R P##check_output_complex_3d(const INT *n, const INT *local_n,                    \
+      const INT *local_n_start, const C *data, MPI_Comm comm);                      \
+  R P##check_output_complex(int rnk_n, const INT *n, const INT *local_n,            \
+      const INT *local_start, const C *data, MPI_Comm comm);                        \
+  R P##check_output_complex_hermitian_3d(const INT *n, const INT *local_n,          \
+      const INT *local_n_start, const C *data, MPI_Comm comm);                      \
+  R P##check_output_complex_hermitian(int rnk_n, const INT *n,                      \
+      const INT *local_n, const INT *local_start, const C *data, MPI_Comm comm);    \
+  R P##check_output_real_3d(const INT *n, const INT *local_n,                       \
+      const INT *local_n_start, const R *data, MPI_Comm comm);                      \
+  R P##check_output_real(int rnk_n, const INT *n, const INT *local_n,               \
+      const INT *local_start, const R *data, MPI_Comm comm);                        \
+  /* layout: api/pfft.h:151-206 (local_size_*), 214-275 (local_block_*) */          \
+  INT P##local_size_dft_3d(const INT *n, MPI_Comm comm_cart, unsigned pfft_flags,   \
+      INT *local_ni, INT *local_i_start, INT *local_no, INT *local_o_start);        \
+  INT P##local_size_dft_r2c_3d(const INT *n, MPI_Comm comm_cart,                    \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  INT P##local_size_dft_c2r_3d(const INT *n, MPI_Comm comm_cart,                    \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  INT P##local_size_dft(int rnk_n, const INT *n, MPI_Comm comm_cart,                \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  INT P##local_size_dft_r2c(int rnk_n, const INT *n, MPI_Comm comm_cart,            \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  INT P##local_size_dft_c2r(int rnk_n, const INT *n, MPI_Comm comm_cart,            \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  INT P##local_size_many_dft(int rnk_n, const INT *n, const INT *ni,                \
+      const INT *no, INT howmany, const INT *iblock, const INT *oblock,             \
+      MPI_Comm comm_cart, unsigned pfft_flags, INT *local_ni,                       \
+      INT *local_i_start, INT *local_no, INT *local_o_start);                       \
+  INT P##local_size_many_dft_r2c(int rnk_n, const INT *n, const INT *ni,            \
+      const INT *no, INT howmany, const INT *iblock, const INT *oblock,             \
+      MPI_Comm comm_cart, unsigned pfft_flags, INT *local_ni,                       \
+      INT *local_i_start, INT *local_no, INT *local_o_start);                       \
+  INT P##local_size_many_dft_c2r(int rnk_n, const INT *n, const INT *ni,            \
+      const INT *no, INT howmany, const INT *iblock, const INT *oblock,             \
+      MPI_Comm comm_cart, unsigned pfft_flags, INT *local_ni,                       \
+      INT *local_i_start, INT *local_no, INT *local_o_start);                       \
+  void P##local_block_dft_3d(const INT *n, MPI_Comm comm_cart, int pid,             \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  void P##local_block_dft_r2c_3d(const INT *n, MPI_Comm comm_cart, int pid,         \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  void P##local_block_dft_c2r_3d(const INT *n, MPI_Comm comm_cart, int pid,         \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  void P##local_block_dft(int rnk_n, const INT *n, MPI_Comm comm_cart, int pid,     \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  void P##local_block_dft_r2c(int rnk_n, const INT *n, MPI_Comm comm_cart,          \
+      int pid, unsigned pfft_flags, INT *local_ni, INT *local_i_start,              \
+      INT *local_no, INT *local_o_start);                                           \
+  void P##local_block_dft_c2r(int rnk_n, const INT *n, MPI_Comm comm_cart,          \
+      int pid, unsigned pfft_flags, INT *local_ni, INT *local_i_start,              \
+      INT *local_no, INT *local_o_start);                                           \
+  void P##local_block_many_dft(int rnk_n, const INT *ni, const INT *no,             \
+      const INT *iblock, const INT *oblock, MPI_Comm comm_cart, int pid,            \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  void P##local_block_many_dft_r2c(int rnk_n, const INT *ni, const INT *no,         \
+      const INT *iblock, const INT *oblock, MPI_Comm comm_cart, int pid,            \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  void P##local_block_many_dft_c2r(int rnk_n, const INT *ni, const INT *no,         \
+      const INT *iblock, const INT *oblock, MPI_Comm comm_cart, int pid,            \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  /* planning: api/pfft.h:277-317 (basic/many), 319-343 (skipped) */                \
+  P##plan P##plan_dft_3d(const INT *n, C *in, C *out, MPI_Comm comm_cart,           \
+      int sign, unsigned pfft_flags);                                               \
+  P##plan P##plan_dft_r2c_3d(const INT *n, R *in, C *out, MPI_Comm comm_cart,       \
+      int sign, unsigned pfft_flags);                                               \
+  P##plan P##plan_dft_c2r_3d(const INT *n, C *in, R *out, MPI_Comm comm_cart,       \
+      int sign, unsigned pfft_flags);                                               \
+  P##plan P##plan_dft(int rnk_n, const INT *n, C *in, C *out, MPI_Comm comm_cart,   \
+      int sign, unsigned pfft_flags);                                               \
+  P##plan P##plan_dft_r2c(int rnk_n, const INT *n, R *in, C *out,                   \
+      MPI_Comm comm_cart, int sign, unsigned pfft_flags);                           \
+  P##plan P##plan_dft_c2r(int rnk_n, const INT *n, C *in, R *out,                   \
+      MPI_Comm comm_cart, int sign, unsigned pfft_flags);                           \
+  P##plan P##plan_many_dft(int rnk_n, const INT *n, const INT *ni, const INT *no,   \
+      INT howmany, const INT *iblock, const INT *oblock, C *in, C *out,             \
+      MPI_Comm comm_cart, int sign, unsigned pfft_flags);                           \
+  P##plan P##plan_many_dft_r2c(int rnk_n, const INT *n, const INT *ni,              \
+      const INT *no, INT howmany, const INT *iblock, const INT *oblock, R *in,      \
+      C *out, MPI_Comm comm_cart, int sign, unsigned pfft_flags);                   \
+  P##plan P##plan_many_dft_c2r(int rnk_n, const INT *n, const INT *ni,              \
+      const INT *no, INT howmany, const INT *iblock, const INT *oblock, C *in,      \
+      R *out, MPI_Comm comm_cart, int sign, unsigned pfft_flags);                   \
+  P##plan P##plan_many_dft_skipped(int rnk_n, const INT *n, const INT *ni,          \
+      const INT *no, INT howmany, const INT *iblock, const INT *oblock,             \
+      const int *skip_trafos, C *in, C *out, MPI_Comm comm_cart, int sign,          \
+      unsigned pfft_flags);                                                         \
+  P##plan P##plan_many_dft_r2c_skipped(int rnk_n, const INT *n, const INT *ni,      \
+      const INT *no, INT howmany, const INT *iblock, const INT *oblock,             \
+      const int *skip_trafos, R *in, C *out, MPI_Comm comm_cart, int sign,          \
+      unsigned pfft_flags);                                                         \
+  P##plan P##plan_many_dft_c2r_skipped(int rnk_n, const INT *n, const INT *ni,      \
+      const INT *no, INT howmany, const INT *iblock, const INT *oblock,             \
+      const int *skip_trafos, C *in, R *out, MPI_Comm comm_cart, int sign,          \
+      unsigned pfft_flags);                                                         \
+  /* integer vector helpers: api/pfft.h:345-359 */                                  \
+  INT  P##prod_INT(int d, const INT *vec);                                          \
+  INT  P##sum_INT(int d, const INT *vec);                                           \
+  int  P##equal_INT(int d, const INT *vec1, const INT *vec2);                       \
+  void P##vcopy_INT(int d, const INT *vec1, INT *vec2);                             \
+  void P##vadd_INT(int d, const INT *vec1, const INT *vec2, INT *sum);              \
+  void P##vsub_INT(int d, const INT *vec1, const INT *vec2, INT *sum);              \
+  /* rank-0 printing: api/pfft.h:416-421 */                                         \
+  void P##vfprintf(MPI_Comm comm, FILE *stream, const char *format, va_list ap);    \
+  void P##fprintf(MPI_Comm comm, FILE *stream, const char *format, ...);            \
+  void P##printf(MPI_Comm comm, const char *format, ...);                           \
+  /* process mesh: api/pfft.h:423-431 */                                            \
+  int P##create_procmesh(int rnk_n, MPI_Comm comm, const int *np,                   \
+      MPI_Comm *comm_cart);                                                         \
+  int P##create_procmesh_1d(MPI_Comm comm, int np0, MPI_Comm *comm_cart_1d);        \
+  int P##create_procmesh_2d(MPI_Comm comm, int np0, int np1,                        \
+      MPI_Comm *comm_cart_2d);
+
+PFFT_AMD_DECLARE_API(pfft_, double, pfft_complex, ptrdiff_t)
+PFFT_AMD_DECLARE_API(pfftf_, float, pfftf_complex, ptrdiff_t)
+
+/* ---------------------------------------------------------------------
+ * Extensions (no counterpart in the reference; they change no existing
+ * symbol).
+ * ------------------------------------------------------------------ */
+
+/* Execute the adjoint (conjugate-transposed) pipeline of a plan: maps an
+ * array shaped like the plan's output to an array shaped like its input.
+ * For r2c plans the adjoint includes the Hermitian weighting (factor 2 on
+ * every half-spectrum mode except the DC and Nyquist planes).  Blocking. */
+void pfft_execute_adjoint(const pfft_plan plan, void *in, void *out);
+void pfftf_execute_adjoint(const pfftf_plan plan, void *in, void *out);
+
+/* Execute `count` plans as one batch (one persistent device launch per
+ * pass kind where the plans allow it).  Blocking. */
+void pfftf_execute_batch(const pfftf_plan *plans, int count);
+void pfft_execute_batch(const pfft_plan *plans, int count);
+
+/* Copies between host and device memory (blocking), for callers that
+ * fill arrays from the host. 0 on success. */
+int pfft_amd_memcpy_to_device(void *dst_dev, const void *src_host, size_t bytes);
+int pfft_amd_memcpy_to_host(void *dst_host, const void *src_dev, size_t bytes);
+/* Library version string. */
+const char *pfft_amd_version(void);
+/* Last error message of the device runtime (empty string if none). */
+const char *pfft_amd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFFT_H */

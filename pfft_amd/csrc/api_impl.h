@@ -1,0 +1,372 @@
+/*
+ * api_impl.h -- the pfft.h entry points for one precision.  Included by
+ * api.c twice, with
+ *   PA(name)  the exported name (pfft_name / pfftf_name)
+ *   R, CPLX   real and complex element types (CPLX is R[2])
+ *   PA_PREC   PFK_F64 or PFK_F32
+ * Each function cites the reference function whose contract it keeps.
+ */
+
+/* ---------------- setup: api/api-basic.c:198-228 ---------------- */
+void PA(init)(void) {
+  if (pfk_init()) pa_error("device initialisation failed: %s", pfk_last_error());
+}
+void PA(cleanup)(void) { (void)pfk_sync(); }
+void PA(plan_with_nthreads)(int nthreads) { g_nthreads = nthreads > 0 ? nthreads : 1; }
+int PA(get_nthreads)(void) { return g_nthreads; }
+
+/* ---------------- memory: kernel/malloc.c:28-43 ---------------- */
+void *PA(malloc)(size_t n) { return pa_alloc(n); }
+R *PA(alloc_real)(size_t n) { return (R *)pa_alloc(sizeof(R) * n); }
+CPLX *PA(alloc_complex)(size_t n) { return (CPLX *)pa_alloc(sizeof(CPLX) * n); }
+void PA(free)(void *p) { pa_dealloc(p); }
+
+/* ---------------- plans: api/api-basic.c:945-991, api/api-adv.c:91-217 ---------------- */
+static PA(plan) PA(mk)(int rnk_n, const INT *n, const INT *ni, const INT *no, INT howmany,
+                       const INT *iblock, const INT *oblock, void *in, void *out,
+                       MPI_Comm comm, int sign, const int *skip, unsigned tf,
+                       unsigned pfft_flags) {
+  return (PA(plan))pa_plan_create(PA_PREC, rnk_n, n, ni, no, howmany, iblock, oblock, in, out,
+                                  comm, sign, skip, tf, pfft_flags);
+}
+
+PA(plan) PA(plan_many_dft)(int rnk_n, const INT *n, const INT *ni, const INT *no, INT howmany,
+                           const INT *iblock, const INT *oblock, CPLX *in, CPLX *out,
+                           MPI_Comm comm_cart, int sign, unsigned pfft_flags) {
+  return PA(mk)(rnk_n, n, ni, no, howmany, iblock, oblock, in, out, comm_cart, sign, NULL, TF_C2C,
+                pfft_flags);
+}
+PA(plan) PA(plan_many_dft_r2c)(int rnk_n, const INT *n, const INT *ni, const INT *no,
+                               INT howmany, const INT *iblock, const INT *oblock, R *in,
+                               CPLX *out, MPI_Comm comm_cart, int sign, unsigned pfft_flags) {
+  return PA(mk)(rnk_n, n, ni, no, howmany, iblock, oblock, in, out, comm_cart, sign, NULL,
+                r2c_flag(pfft_flags), pfft_flags);
+}
+PA(plan) PA(plan_many_dft_c2r)(int rnk_n, const INT *n, const INT *ni, const INT *no,
+                               INT howmany, const INT *iblock, const INT *oblock, CPLX *in,
+                               R *out, MPI_Comm comm_cart, int sign, unsigned pfft_flags) {
+  return PA(mk)(rnk_n, n, ni, no, howmany, iblock, oblock, in, out, comm_cart, sign, NULL,
+                c2r_flag(pfft_flags), pfft_flags);
+}
+PA(plan) PA(plan_many_dft_skipped)(int rnk_n, const INT *n, const INT *ni, const INT *no,
+                                   INT howmany, const INT *iblock, const INT *oblock,
+                                   const int *skip_trafos, CPLX *in, CPLX *out,
+                                   MPI_Comm comm_cart, int sign, unsigned pfft_flags) {
+  return PA(mk)(rnk_n, n, ni, no, howmany, iblock, oblock, in, out, comm_cart, sign, skip_trafos,
+                TF_C2C, pfft_flags);
+}
+PA(plan) PA(plan_many_dft_r2c_skipped)(int rnk_n, const INT *n, const INT *ni, const INT *no,
+                                       INT howmany, const INT *iblock, const INT *oblock,
+                                       const int *skip_trafos, R *in, CPLX *out,
+                                       MPI_Comm comm_cart, int sign, unsigned pfft_flags) {
+  return PA(mk)(rnk_n, n, ni, no, howmany, iblock, oblock, in, out, comm_cart, sign, skip_trafos,
+                r2c_flag(pfft_flags), pfft_flags);
+}
+PA(plan) PA(plan_many_dft_c2r_skipped)(int rnk_n, const INT *n, const INT *ni, const INT *no,
+                                       INT howmany, const INT *iblock, const INT *oblock,
+                                       const int *skip_trafos, CPLX *in, R *out,
+                                       MPI_Comm comm_cart, int sign, unsigned pfft_flags) {
+  return PA(mk)(rnk_n, n, ni, no, howmany, iblock, oblock, in, out, comm_cart, sign, skip_trafos,
+                c2r_flag(pfft_flags), pfft_flags);
+}
+PA(plan) PA(plan_dft)(int rnk_n, const INT *n, CPLX *in, CPLX *out, MPI_Comm comm_cart,
+                      int sign, unsigned pfft_flags) {
+  return PA(plan_many_dft)(rnk_n, n, n, n, 1, PFFT_DEFAULT_BLOCKS, PFFT_DEFAULT_BLOCKS, in, out,
+                           comm_cart, sign, pfft_flags);
+}
+PA(plan) PA(plan_dft_r2c)(int rnk_n, const INT *n, R *in, CPLX *out, MPI_Comm comm_cart,
+                          int sign, unsigned pfft_flags) {
+  return PA(plan_many_dft_r2c)(rnk_n, n, n, n, 1, PFFT_DEFAULT_BLOCKS, PFFT_DEFAULT_BLOCKS, in,
+                               out, comm_cart, sign, pfft_flags);
+}
+PA(plan) PA(plan_dft_c2r)(int rnk_n, const INT *n, CPLX *in, R *out, MPI_Comm comm_cart,
+                          int sign, unsigned pfft_flags) {
+  return PA(plan_many_dft_c2r)(rnk_n, n, n, n, 1, PFFT_DEFAULT_BLOCKS, PFFT_DEFAULT_BLOCKS, in,
+                               out, comm_cart, sign, pfft_flags);
+}
+PA(plan) PA(plan_dft_3d)(const INT *n, CPLX *in, CPLX *out, MPI_Comm comm_cart, int sign,
+                         unsigned pfft_flags) {
+  return PA(plan_dft)(3, n, in, out, comm_cart, sign, pfft_flags);
+}
+PA(plan) PA(plan_dft_r2c_3d)(const INT *n, R *in, CPLX *out, MPI_Comm comm_cart, int sign,
+                             unsigned pfft_flags) {
+  return PA(plan_dft_r2c)(3, n, in, out, comm_cart, sign, pfft_flags);
+}
+PA(plan) PA(plan_dft_c2r_3d)(const INT *n, CPLX *in, R *out, MPI_Comm comm_cart, int sign,
+                             unsigned pfft_flags) {
+  return PA(plan_dft_c2r)(3, n, in, out, comm_cart, sign, pfft_flags);
+}
+
+/* ---------------- execution: api/api-basic.c:1109-1147, 239-249 ---------------- */
+void PA(execute)(const PA(plan) plan) {
+  if (!plan) { fprintf(stderr, "!!! Error: Can not execute PFFT Plan == NULL !!!\n"); return; }
+  pa_plan_execute((const pa_plan *)plan, pa_plan_in((const pa_plan *)plan),
+                  pa_plan_out((const pa_plan *)plan));
+}
+void PA(execute_dft)(const PA(plan) plan, CPLX *in, CPLX *out) {
+  if (!plan) { fprintf(stderr, "!!! Error: Can not execute PFFT Plan == NULL !!!\n"); return; }
+  pa_plan_execute((const pa_plan *)plan, in, out);
+}
+void PA(execute_dft_r2c)(const PA(plan) plan, R *in, CPLX *out) {
+  if (!plan) { fprintf(stderr, "!!! Error: Can not execute PFFT Plan == NULL !!!\n"); return; }
+  pa_plan_execute((const pa_plan *)plan, in, out);
+}
+void PA(execute_dft_c2r)(const PA(plan) plan, CPLX *in, R *out) {
+  if (!plan) { fprintf(stderr, "!!! Error: Can not execute PFFT Plan == NULL !!!\n"); return; }
+  pa_plan_execute((const pa_plan *)plan, in, out);
+}
+void PA(execute_r2r)(const PA(plan) plan, R *in, R *out) {
+  (void)in; (void)out;
+  if (!plan) { fprintf(stderr, "!!! Error: Can not execute PFFT Plan == NULL !!!\n"); return; }
+  pa_error("r2r transforms are not supported");
+}
+void PA(execute_adjoint)(const PA(plan) plan, void *in, void *out) {
+  if (!plan) { fprintf(stderr, "!!! Error: Can not execute PFFT Plan == NULL !!!\n"); return; }
+  pa_plan_execute_adjoint((const pa_plan *)plan, in, out);
+}
+void PA(execute_batch)(const PA(plan) *plans, int count) {
+  pa_plans_execute_batch((const pa_plan *const *)plans, count);
+}
+void PA(destroy_plan)(PA(plan) plan) {
+  if (!plan) {
+    fprintf(stderr, "!!! Error: Can not finalize PFFT Plan == NULL !!!\n");
+    return;
+  }
+  pa_plan_destroy((pa_plan *)plan);
+}
+
+/* ---------------- layout: api/api-basic.c:736-773,887-941,993-1039; api/api-adv.c:30-88,327-385 ---------------- */
+static INT PA(ls)(int rnk_n, const INT *n, const INT *ni, const INT *no, INT howmany,
+                  const INT *iblock, const INT *oblock, MPI_Comm comm, unsigned tf,
+                  unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  pa_mesh m;
+  if (pa_mesh_of(comm, &m)) return 0;
+  return pa_local_size(rnk_n, n, ni, no, howmany, iblock, oblock, &m, tf, pfft_flags, lni, lis,
+                       lno, los);
+}
+INT PA(local_size_many_dft)(int rnk_n, const INT *n, const INT *ni, const INT *no, INT howmany,
+                            const INT *iblock, const INT *oblock, MPI_Comm comm_cart,
+                            unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  return PA(ls)(rnk_n, n, ni, no, howmany, iblock, oblock, comm_cart, TF_C2C, pfft_flags, lni,
+                lis, lno, los);
+}
+INT PA(local_size_many_dft_r2c)(int rnk_n, const INT *n, const INT *ni, const INT *no,
+                                INT howmany, const INT *iblock, const INT *oblock,
+                                MPI_Comm comm_cart, unsigned pfft_flags, INT *lni, INT *lis,
+                                INT *lno, INT *los) {
+  return PA(ls)(rnk_n, n, ni, no, howmany, iblock, oblock, comm_cart, r2c_flag(pfft_flags),
+                pfft_flags, lni, lis, lno, los);
+}
+INT PA(local_size_many_dft_c2r)(int rnk_n, const INT *n, const INT *ni, const INT *no,
+                                INT howmany, const INT *iblock, const INT *oblock,
+                                MPI_Comm comm_cart, unsigned pfft_flags, INT *lni, INT *lis,
+                                INT *lno, INT *los) {
+  return PA(ls)(rnk_n, n, ni, no, howmany, iblock, oblock, comm_cart, c2r_flag(pfft_flags),
+                pfft_flags, lni, lis, lno, los);
+}
+INT PA(local_size_dft)(int rnk_n, const INT *n, MPI_Comm comm_cart, unsigned pfft_flags, INT *lni,
+                       INT *lis, INT *lno, INT *los) {
+  return PA(local_size_many_dft)(rnk_n, n, n, n, 1, PFFT_DEFAULT_BLOCKS, PFFT_DEFAULT_BLOCKS,
+                                 comm_cart, pfft_flags, lni, lis, lno, los);
+}
+INT PA(local_size_dft_r2c)(int rnk_n, const INT *n, MPI_Comm comm_cart, unsigned pfft_flags,
+                           INT *lni, INT *lis, INT *lno, INT *los) {
+  return PA(local_size_many_dft_r2c)(rnk_n, n, n, n, 1, PFFT_DEFAULT_BLOCKS,
+                                     PFFT_DEFAULT_BLOCKS, comm_cart, pfft_flags, lni, lis, lno,
+                                     los);
+}
+INT PA(local_size_dft_c2r)(int rnk_n, const INT *n, MPI_Comm comm_cart, unsigned pfft_flags,
+                           INT *lni, INT *lis, INT *lno, INT *los) {
+  return PA(local_size_many_dft_c2r)(rnk_n, n, n, n, 1, PFFT_DEFAULT_BLOCKS,
+                                     PFFT_DEFAULT_BLOCKS, comm_cart, pfft_flags, lni, lis, lno,
+                                     los);
+}
+INT PA(local_size_dft_3d)(const INT *n, MPI_Comm comm_cart, unsigned pfft_flags, INT *lni,
+                          INT *lis, INT *lno, INT *los) {
+  return PA(local_size_dft)(3, n, comm_cart, pfft_flags, lni, lis, lno, los);
+}
+INT PA(local_size_dft_r2c_3d)(const INT *n, MPI_Comm comm_cart, unsigned pfft_flags, INT *lni,
+                              INT *lis, INT *lno, INT *los) {
+  return PA(local_size_dft_r2c)(3, n, comm_cart, pfft_flags, lni, lis, lno, los);
+}
+INT PA(local_size_dft_c2r_3d)(const INT *n, MPI_Comm comm_cart, unsigned pfft_flags, INT *lni,
+                              INT *lis, INT *lno, INT *los) {
+  return PA(local_size_dft_c2r)(3, n, comm_cart, pfft_flags, lni, lis, lno, los);
+}
+
+static void PA(lb)(int rnk_n, const INT *ni, const INT *no, const INT *iblock, const INT *oblock,
+                   MPI_Comm comm, int pid, unsigned tf, unsigned pfft_flags, INT *lni, INT *lis,
+                   INT *lno, INT *los) {
+  pa_mesh m;
+  if (pa_mesh_of(comm, &m)) return;
+  pa_local_block(rnk_n, ni, no, iblock, oblock, &m, pid, tf, pfft_flags, lni, lis, lno, los);
+}
+void PA(local_block_many_dft)(int rnk_n, const INT *ni, const INT *no, const INT *iblock,
+                              const INT *oblock, MPI_Comm comm_cart, int pid,
+                              unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(lb)(rnk_n, ni, no, iblock, oblock, comm_cart, pid, TF_C2C, pfft_flags, lni, lis, lno, los);
+}
+void PA(local_block_many_dft_r2c)(int rnk_n, const INT *ni, const INT *no, const INT *iblock,
+                                  const INT *oblock, MPI_Comm comm_cart, int pid,
+                                  unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(lb)(rnk_n, ni, no, iblock, oblock, comm_cart, pid, r2c_flag(pfft_flags), pfft_flags, lni,
+         lis, lno, los);
+}
+void PA(local_block_many_dft_c2r)(int rnk_n, const INT *ni, const INT *no, const INT *iblock,
+                                  const INT *oblock, MPI_Comm comm_cart, int pid,
+                                  unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(lb)(rnk_n, ni, no, iblock, oblock, comm_cart, pid, c2r_flag(pfft_flags), pfft_flags, lni,
+         lis, lno, los);
+}
+void PA(local_block_dft)(int rnk_n, const INT *n, MPI_Comm comm_cart, int pid,
+                         unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(local_block_many_dft)(rnk_n, n, n, PFFT_DEFAULT_BLOCKS, PFFT_DEFAULT_BLOCKS, comm_cart, pid,
+                           pfft_flags, lni, lis, lno, los);
+}
+void PA(local_block_dft_r2c)(int rnk_n, const INT *n, MPI_Comm comm_cart, int pid,
+                             unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(local_block_many_dft_r2c)(rnk_n, n, n, PFFT_DEFAULT_BLOCKS, PFFT_DEFAULT_BLOCKS, comm_cart,
+                               pid, pfft_flags, lni, lis, lno, los);
+}
+void PA(local_block_dft_c2r)(int rnk_n, const INT *n, MPI_Comm comm_cart, int pid,
+                             unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(local_block_many_dft_c2r)(rnk_n, n, n, PFFT_DEFAULT_BLOCKS, PFFT_DEFAULT_BLOCKS, comm_cart,
+                               pid, pfft_flags, lni, lis, lno, los);
+}
+void PA(local_block_dft_3d)(const INT *n, MPI_Comm comm_cart, int pid, unsigned pfft_flags,
+                            INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(local_block_dft)(3, n, comm_cart, pid, pfft_flags, lni, lis, lno, los);
+}
+void PA(local_block_dft_r2c_3d)(const INT *n, MPI_Comm comm_cart, int pid, unsigned pfft_flags,
+                                INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(local_block_dft_r2c)(3, n, comm_cart, pid, pfft_flags, lni, lis, lno, los);
+}
+void PA(local_block_dft_c2r_3d)(const INT *n, MPI_Comm comm_cart, int pid, unsigned pfft_flags,
+                                INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(local_block_dft_c2r)(3, n, comm_cart, pid, pfft_flags, lni, lis, lno, los);
+}
+
+/* ---------------- process mesh: kernel/procmesh.c:36-89 ---------------- */
+int PA(create_procmesh)(int rnk_n, MPI_Comm comm, const int *np, MPI_Comm *comm_cart) {
+  return pa_create_procmesh(rnk_n, comm, np, comm_cart);
+}
+int PA(create_procmesh_1d)(MPI_Comm comm, int np0, MPI_Comm *comm_cart_1d) {
+  int np[1] = {np0};
+  return pa_create_procmesh(1, comm, np, comm_cart_1d);
+}
+int PA(create_procmesh_2d)(MPI_Comm comm, int np0, int np1, MPI_Comm *comm_cart_2d) {
+  int np[2] = {np0, np1};
+  return pa_create_procmesh(2, comm, np, comm_cart_2d);
+}
+
+/* ---------------- integer vectors: util/util.c:103-173 ---------------- */
+INT PA(prod_INT)(int d, const INT *vec) {
+  INT p = 1;
+  for (int t = 0; t < d; t++) p *= vec[t];
+  return p;
+}
+INT PA(sum_INT)(int d, const INT *vec) {
+  INT s = 0;
+  for (int t = 0; t < d; t++) s += vec[t];
+  return s;
+}
+int PA(equal_INT)(int d, const INT *v1, const INT *v2) {
+  int eq = 1;
+  for (int t = 0; t < d; t++) eq *= (v1[t] == v2[t]);
+  return eq;
+}
+void PA(vcopy_INT)(int d, const INT *v1, INT *v2) {
+  for (int t = 0; t < d; t++) v2[t] = v1[t];
+}
+void PA(vadd_INT)(int d, const INT *v1, const INT *v2, INT *sum) {
+  for (int t = 0; t < d; t++) sum[t] = v1[t] + v2[t];
+}
+void PA(vsub_INT)(int d, const INT *v1, const INT *v2, INT *sum) {
+  for (int t = 0; t < d; t++) sum[t] = v1[t] - v2[t];
+}
+
+/* ---------------- rank-0 printing: util/util.c:175-201 ---------------- */
+void PA(vfprintf)(MPI_Comm comm, FILE *stream, const char *format, va_list ap) {
+  (void)comm;
+  vfprintf(stream, format, ap);
+}
+void PA(fprintf)(MPI_Comm comm, FILE *stream, const char *format, ...) {
+  va_list ap;
+  va_start(ap, format);
+  PA(vfprintf)(comm, stream, format, ap);
+  va_end(ap);
+}
+void PA(printf)(MPI_Comm comm, const char *format, ...) {
+  va_list ap;
+  va_start(ap, format);
+  PA(vfprintf)(comm, stdout, format, ap);
+  va_end(ap);
+}
+
+/* ---------------- test-data tools: api/api-basic.c:60-189, 295-479 ---------------- */
+void PA(init_input_complex)(int rnk_n, const INT *n, const INT *ln, const INT *ls, CPLX *data) {
+  pa_tool_array(PA_PREC, PA_TOOL_INIT, AT_COMPLEX, rnk_n, n, ln, ls, data);
+}
+void PA(init_input_complex_hermitian)(int rnk_n, const INT *n, const INT *ln, const INT *ls,
+                                      CPLX *data) {
+  pa_tool_array(PA_PREC, PA_TOOL_INIT, AT_HERMITIAN, rnk_n, n, ln, ls, data);
+}
+void PA(init_input_real)(int rnk_n, const INT *n, const INT *ln, const INT *ls, R *data) {
+  pa_tool_array(PA_PREC, PA_TOOL_INIT, AT_REAL, rnk_n, n, ln, ls, data);
+}
+void PA(clear_input_complex)(int rnk_n, const INT *n, const INT *ln, const INT *ls, CPLX *data) {
+  pa_tool_array(PA_PREC, PA_TOOL_CLEAR, AT_COMPLEX, rnk_n, n, ln, ls, data);
+}
+void PA(clear_input_complex_hermitian)(int rnk_n, const INT *n, const INT *ln, const INT *ls,
+                                       CPLX *data) {
+  pa_tool_array(PA_PREC, PA_TOOL_CLEAR, AT_HERMITIAN, rnk_n, n, ln, ls, data);
+}
+void PA(clear_input_real)(int rnk_n, const INT *n, const INT *ln, const INT *ls, R *data) {
+  pa_tool_array(PA_PREC, PA_TOOL_CLEAR, AT_REAL, rnk_n, n, ln, ls, data);
+}
+R PA(check_output_complex)(int rnk_n, const INT *n, const INT *ln, const INT *ls,
+                           const CPLX *data, MPI_Comm comm) {
+  (void)comm;
+  return (R)pa_tool_array(PA_PREC, PA_TOOL_CHECK, AT_COMPLEX, rnk_n, n, ln, ls, (void *)data);
+}
+R PA(check_output_complex_hermitian)(int rnk_n, const INT *n, const INT *ln, const INT *ls,
+                                     const CPLX *data, MPI_Comm comm) {
+  (void)comm;
+  return (R)pa_tool_array(PA_PREC, PA_TOOL_CHECK, AT_HERMITIAN, rnk_n, n, ln, ls, (void *)data);
+}
+R PA(check_output_real)(int rnk_n, const INT *n, const INT *ln, const INT *ls, const R *data,
+                        MPI_Comm comm) {
+  (void)comm;
+  return (R)pa_tool_array(PA_PREC, PA_TOOL_CHECK, AT_REAL, rnk_n, n, ln, ls, (void *)data);
+}
+void PA(init_input_complex_3d)(const INT *n, const INT *ln, const INT *ls, CPLX *data) {
+  PA(init_input_complex)(3, n, ln, ls, data);
+}
+void PA(init_input_complex_hermitian_3d)(const INT *n, const INT *ln, const INT *ls, CPLX *data) {
+  PA(init_input_complex_hermitian)(3, n, ln, ls, data);
+}
+void PA(init_input_real_3d)(const INT *n, const INT *ln, const INT *ls, R *data) {
+  PA(init_input_real)(3, n, ln, ls, data);
+}
+void PA(clear_input_complex_3d)(const INT *n, const INT *ln, const INT *ls, CPLX *data) {
+  PA(clear_input_complex)(3, n, ln, ls, data);
+}
+void PA(clear_input_complex_hermitian_3d)(const INT *n, const INT *ln, const INT *ls,
+                                         CPLX *data) {
+  PA(clear_input_complex_hermitian)(3, n, ln, ls, data);
+}
+void PA(clear_input_real_3d)(const INT *n, const INT *ln, const INT *ls, R *data) {
+  PA(clear_input_real)(3, n, ln, ls, data);
+}
+R PA(check_output_complex_3d)(const INT *n, const INT *ln, const INT *ls, const CPLX *data,
+                              MPI_Comm comm) {
+  return PA(check_output_complex)(3, n, ln, ls, data, comm);
+}
+R PA(check_output_complex_hermitian_3d)(const INT *n, const INT *ln, const INT *ls,
+                                        const CPLX *data, MPI_Comm comm) {
+  return PA(check_output_complex_hermitian)(3, n, ln, ls, data, comm);
+}
+R PA(check_output_real_3d)(const INT *n, const INT *ln, const INT *ls, const R *data,
+                           MPI_Comm comm) {
+  return PA(check_output_real)(3, n, ln, ls, data, comm);
+}

@@ -1,0 +1,74 @@
+/*
+ * host.h -- internal declarations of the C host library (planner,
+ * executor, API tools).  Not installed; the public surface is
+ * include/pfft.h.
+ */
+#ifndef PFFT_AMD_HOST_H
+#define PFFT_AMD_HOST_H 1
+
+#include <stddef.h>
+#include <stdint.h>
+#include "pfft.h"
+#include "pfk.h"
+
+typedef ptrdiff_t INT;
+
+#define PA_MAXRNK 8
+
+/* internal transform flags: same meaning as the reference's PFFTI_TRAFO_*
+ * (kernel/ipfft.h:137-163) */
+#define TF_C2C      (1U << 0)
+#define TF_R2C      (1U << 1)
+#define TF_C2R      (1U << 2)
+#define TF_R2R      (0x7F8U)     /* RE00..RO11, bits 3..10 */
+#define TF_SKIP     (1U << 11)
+#define TF_PHANTOM  (1U << 12)
+#define TF_PADDED   (1U << 13)
+#define TF_RDFT     (TF_R2C | TF_C2R)
+
+#define PA_MAX(a, b) ((a) > (b) ? (a) : (b))
+
+/* ---- process mesh (comm.c) ---- */
+typedef struct pa_mesh {
+  int rnk_pm;          /* Cartesian dimension (a non-Cartesian comm is a 1-d mesh) */
+  int np[PA_MAXRNK];   /* processes per mesh dimension (all 1 on one GPU) */
+  int size;            /* total number of processes */
+} pa_mesh;
+
+/* Fills *m for `comm`; returns 0 if the communicator is usable by this
+ * one-process library. */
+int pa_mesh_of(MPI_Comm comm, pa_mesh *m);
+int pa_create_procmesh(int rnk, MPI_Comm comm, const int *np, MPI_Comm *comm_cart);
+
+/* ---- layout machinery (layout.c) ---- */
+INT pa_local_size(int rnk_n, const INT *n, const INT *ni, const INT *no, INT howmany,
+                  const INT *iblock, const INT *oblock, const pa_mesh *m,
+                  unsigned trafo_flag, unsigned pfft_flags,
+                  INT *lni, INT *lis, INT *lno, INT *los);
+void pa_local_block(int rnk_n, const INT *ni, const INT *no, const INT *iblock,
+                    const INT *oblock, const pa_mesh *m, int pid, unsigned trafo_flag,
+                    unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los);
+INT pa_physical_size_1d(INT n, unsigned trafo_flag);
+
+/* ---- plans (plan.c) ---- */
+typedef struct pa_plan pa_plan;
+pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const INT *no,
+                        INT howmany, const INT *iblock, const INT *oblock, void *in,
+                        void *out, MPI_Comm comm, int sign, const int *skip_trafos,
+                        unsigned trafo_flag, unsigned pfft_flags);
+void pa_plan_execute(const pa_plan *p, void *in, void *out);
+void pa_plan_execute_adjoint(const pa_plan *p, void *in, void *out);
+void pa_plan_destroy(pa_plan *p);
+void *pa_plan_in(const pa_plan *p);
+void *pa_plan_out(const pa_plan *p);
+void pa_plans_execute_batch(const pa_plan *const *plans, int count);
+
+/* ---- twiddles (twiddle.c) ---- */
+/* exp(sign*2*pi*i*m/N) for m in [0, count), rounded to double, computed in
+ * long double with exact integer quadrant reduction. */
+void pa_twiddles(long N, long count, int sign, double *out_interleaved);
+
+/* ---- errors ---- */
+void pa_error(const char *fmt, ...);
+
+#endif

@@ -1,0 +1,109 @@
+/*
+ * pfk.h -- thin internal C ABI between the C host library (planner,
+ * executor, API) and the hand-written HIP kernels for gfx950.
+ *
+ * Everything here is plain C: plain pointers, sizes and a POD pass
+ * descriptor.  No HIP or torch types cross this boundary, so the host
+ * side (the .c files of pfft_amd/csrc) is compiled by gcc and only the .hip files
+ * by hipcc.
+ *
+ * A "pass" transforms one logical dimension of a strided multi-dimensional
+ * array: for every line along that dimension it
+ *   1. loads nin elements (stride sin), multiplies element j by the
+ *      input modulation factor (+-1), and embeds them at FFT index zin+j
+ *      of a zero line of length n  (the reference's ousam embed,
+ *      kernel/ousample.c:391-456, and twiddle_input, api/api-basic.c:1186);
+ *   2. computes the unnormalised length-n DFT with the given sign
+ *      (FFTW convention: sign -1 forward; kernel/sertrafo.c:489-554);
+ *      kind R2C / C2R are FFTW's half-spectrum real transforms;
+ *   3. keeps nout outputs starting at FFT index zout (the reference's
+ *      ousam trunc, kernel/ousample.c:461-506), multiplies output k' by the
+ *      output modulation factor (api/api-basic.c:1242) and stores them with
+ *      stride sout.
+ * The lines are indexed by up to PFK_MAXB batch dimensions plus one "tile"
+ * dimension whose consecutive values are processed together by one
+ * workgroup so that global loads/stores coalesce along it.
+ */
+#ifndef PFK_H
+#define PFK_H 1
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFK_MAXB 6
+#define PFK_MAXSTAGES 24
+
+enum pfk_kind { PFK_C2C = 0, PFK_R2C = 1, PFK_C2R = 2, PFK_COPY = 3 };
+enum pfk_prec { PFK_F64 = 0, PFK_F32 = 1 };
+
+/* +-1 modulation of a line index:
+ *   f(j) = (on && j+start < half) ? ((j+start) odd ? -1 : 1) * extra : 1  */
+typedef struct pfk_mod {
+  int32_t on;
+  int32_t extra;      /* +1 or -1 */
+  int64_t start;
+  int64_t half;
+} pfk_mod;
+
+typedef struct pfk_pass {
+  int32_t kind;       /* enum pfk_kind */
+  int32_t prec;       /* enum pfk_prec */
+  int32_t sign;       /* -1 or +1 */
+  int32_t nstages;    /* radix factorisation of the computed FFT length */
+  int32_t radix[PFK_MAXSTAGES];
+  int64_t n;          /* logical transform length */
+  int64_t nf;         /* computed complex FFT length (n, or n/2 for even r2c/c2r) */
+  int64_t nin, zin;   /* input count along the line and embed offset */
+  int64_t nout, zout; /* kept output count and first kept index */
+  int64_t sin, sout;  /* line strides, in elements of the src/dst type */
+  pfk_mod imod, omod;
+  int32_t nb;         /* number of non-tile batch dims */
+  int32_t hw;         /* Hermitian weights: 0 none; 1 c2r input x1/2 off DC/Nyquist;
+                         2 r2c output x2 off DC/Nyquist (adjoint passes) */
+  int64_t bext[PFK_MAXB], bsin[PFK_MAXB], bsout[PFK_MAXB];
+  int64_t text, tsin, tsout;   /* tile dimension (text==1: none) */
+  /* device twiddle tables (owned by the host plan):
+   *   tw  : W_nf^m = exp(sign*2*pi*i*m/nf), m in [0,nf)
+   *   rtw : W_n^k  = exp(sign*2*pi*i*k/n),  k in [0,n/2] (even r2c/c2r only) */
+  const void *tw;
+  const void *rtw;
+  /* element counts of the src/dst buffers, for host-side bounds checks */
+  int64_t src_elems, dst_elems;
+} pfk_pass;
+
+/* Device/runtime plumbing.  All return 0 on success, nonzero on error. */
+int  pfk_init(void);
+int  pfk_device_count(void);
+int  pfk_malloc(void **p, size_t bytes);
+int  pfk_malloc_host(void **p, size_t bytes);
+int  pfk_free(void *p);
+int  pfk_free_host(void *p);
+/* 1 if p is device (or managed) memory the GPU can address directly */
+int  pfk_is_device_ptr(const void *p);
+int  pfk_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int  pfk_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int  pfk_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int  pfk_memset(void *dst, int v, size_t bytes);
+int  pfk_sync(void);
+const char *pfk_last_error(void);
+
+/* Launch one pass on the library stream (asynchronous). */
+int  pfk_launch_pass(const pfk_pass *p, const void *src, void *dst);
+
+/* Timing helpers on the library stream (HIP events). */
+int  pfk_event_create(void **ev);
+int  pfk_event_record(void *ev);
+int  pfk_event_elapsed_ms(void *ev0, void *ev1, float *ms);
+int  pfk_event_destroy(void *ev);
+
+/* Complex conjugation / scaling utility kernels on device arrays. */
+int  pfk_scale(void *data, int prec, int64_t nreal, double factor);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

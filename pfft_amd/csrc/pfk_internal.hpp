@@ -1,0 +1,28 @@
+// pfk_internal.hpp -- shared definitions of the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "pfk.h"
+
+namespace pfk {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return cpx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+extern hipStream_t g_stream;
+extern int g_max_lds;
+void set_error(const char *what, hipError_t e);
+void set_error_msg(const char *msg);
+
+// Specialised kernels: return 0 if the pass was launched, nonzero if the
+// pass shape is not covered (the caller then uses the general kernel).
+int launch_fast(const pfk_pass *p, const void *src, void *dst);
+int launch_generic(const pfk_pass *p, const void *src, void *dst);
+
+}  // namespace pfk

@@ -1,0 +1,533 @@
+/*
+ * plan.c -- planning and execution of the transform pipeline on one GPU.
+ *
+ * What a plan computes is the reference's execute_full
+ * (api/api-basic.c:1044-1107) on a one-process mesh:
+ *   conjugate (c2r forward) -> twiddle_input (SHIFTED_OUT) -> per step:
+ *   embed -> serial FFT -> trunc, with local/global transposes ->
+ *   twiddle_output (SHIFTED_IN) -> conjugate (r2c backward).
+ * On one process every global transpose is an identity and every local
+ * transpose is a permutation, so the whole chain is, per dimension d,
+ *   y = M_out,d . Trunc_d . DFT_d . Embed_d . M_in,d  x
+ * applied along d, plus a change of memory order between the input and
+ * output layouts.  The plan therefore builds one device pass per
+ * dimension (pfk.h), folding the +-1 shift modulations, the zero-pad and
+ * the truncation into that pass's loads and stores, and the layout change
+ * into the pass whose transform axis is contiguous on the changed side.
+ * The two conjugations cancel the sign swaps the reference makes for
+ * c2r forward / r2c backward (partrafo.c:426-443), so each pass simply
+ * uses the user's sign.
+ *
+ * Pass order: r2c transforms the last dimension first (its input is
+ * real), c2r last; a TRANSPOSED_IN plan transforms dimension 0 first and
+ * reads the transposed input in that pass; otherwise dimension 0 is done
+ * last and, for TRANSPOSED_OUT, writes the transposed output.
+ */
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "host.h"
+
+enum { ROLE_IN = 0, ROLE_OUT = 1, ROLE_S0 = 2, ROLE_S1 = 3 };
+
+typedef struct arrgeo {
+  int rnk;
+  int order[PA_MAXRNK];   /* memory order of the logical dims, outermost first */
+  INT ext[PA_MAXRNK];     /* logical extents (element counts) */
+  INT phys_last;          /* physical extent of dim rnk-1 (padding), 0 if = ext */
+  INT h;                  /* interleaved tuple (howmany), innermost */
+  int is_real;
+} arrgeo;
+
+struct pa_plan {
+  int prec;
+  int rnk_n;
+  INT n[PA_MAXRNK], ni[PA_MAXRNK], no[PA_MAXRNK];
+  INT howmany;
+  unsigned trafo_flag, pfft_flags, transp;
+  int sign;
+  int skip[PA_MAXRNK];
+  INT local_ni[PA_MAXRNK], local_i_start[PA_MAXRNK];
+  INT local_no[PA_MAXRNK], local_o_start[PA_MAXRNK];
+  void *in, *out;
+  arrgeo gin, gout;
+  int64_t in_elems, out_elems;   /* in scalar elements of the array's type */
+  size_t in_esz, out_esz;        /* bytes per element */
+  int npass;
+  pfk_pass pass[PA_MAXRNK];
+  int src_role[PA_MAXRNK], dst_role[PA_MAXRNK];
+  void *scratch[2];
+  size_t scratch_bytes[2];
+  const void *tw_adj[PA_MAXRNK], *rtw_adj[PA_MAXRNK];   /* opposite-sign tables */
+  void *tw_bufs[4 * PA_MAXRNK];
+  int ntw;
+};
+
+void pa_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  fprintf(stderr, "PFFT ERROR: ");
+  vfprintf(stderr, fmt, ap);
+  fprintf(stderr, "\n");
+  va_end(ap);
+}
+
+static int64_t geo_strides(const arrgeo *a, INT *stride) {
+  INT run = a->h;
+  for (int k = a->rnk - 1; k >= 0; --k) {
+    const int d = a->order[k];
+    stride[d] = run;
+    run *= (d == a->rnk - 1 && a->phys_last) ? a->phys_last : a->ext[d];
+  }
+  return (int64_t)run;
+}
+
+static void order_identity(int rnk, int *order) {
+  for (int t = 0; t < rnk; t++) order[t] = t;
+}
+
+/* transposed decomposition n1 x .. x n_r x n0 x n_{r+1} x .. (doc/tutorial.tex:319-321) */
+static void order_transposed(int rnk, int r, int *order) {
+  int k = 0;
+  for (int t = 1; t <= r; t++) order[k++] = t;
+  order[k++] = 0;
+  for (int t = r + 1; t < rnk; t++) order[k++] = t;
+}
+
+static int same_geo(const arrgeo *a, const arrgeo *b) {
+  if (a->rnk != b->rnk || a->h != b->h || a->is_real != b->is_real) return 0;
+  INT sa[PA_MAXRNK], sb[PA_MAXRNK];
+  geo_strides(a, sa);
+  geo_strides(b, sb);
+  for (int t = 0; t < a->rnk; t++)
+    if (a->ext[t] != b->ext[t] || sa[t] != sb[t]) return 0;
+  return 1;
+}
+
+/* small-prime factorisation of the computed FFT length into radices;
+ * 4s first, then 2, then odd primes */
+static int factorize(INT nf, int32_t *radix) {
+  int ns = 0;
+  INT m = nf;
+  while (m % 4 == 0 && ns < PFK_MAXSTAGES) { radix[ns++] = 4; m /= 4; }
+  while (m % 2 == 0 && ns < PFK_MAXSTAGES) { radix[ns++] = 2; m /= 2; }
+  for (INT p = 3; m > 1 && ns < PFK_MAXSTAGES; p += 2)
+    while (m % p == 0 && ns < PFK_MAXSTAGES) { radix[ns++] = (int32_t)p; m /= p; }
+  if (m != 1) return -1;
+  return ns;
+}
+
+static const void *make_twiddles(pa_plan *p, INT N, INT count, int sign) {
+  double *h = (double *)malloc(sizeof(double) * 2 * (size_t)(count > 0 ? count : 1));
+  if (!h) return NULL;
+  pa_twiddles((long)N, (long)count, sign, h);
+  void *d = NULL;
+  size_t bytes;
+  if (p->prec == PFK_F64) {
+    bytes = sizeof(double) * 2 * (size_t)count;
+    if (pfk_malloc(&d, bytes) || pfk_memcpy_h2d(d, h, bytes)) { free(h); return NULL; }
+  } else {
+    float *f = (float *)malloc(sizeof(float) * 2 * (size_t)count);
+    for (INT i = 0; i < 2 * count; i++) f[i] = (float)h[i];
+    bytes = sizeof(float) * 2 * (size_t)count;
+    if (pfk_malloc(&d, bytes) || pfk_memcpy_h2d(d, f, bytes)) { free(f); free(h); return NULL; }
+    free(f);
+  }
+  free(h);
+  p->tw_bufs[p->ntw++] = d;
+  return d;
+}
+
+typedef struct bdim { INT ext, s_src, s_dst; } bdim;
+
+static int cmp_bdim(const void *a, const void *b) {
+  const bdim *x = (const bdim *)a, *y = (const bdim *)b;
+  if (x->s_src != y->s_src) return (x->s_src > y->s_src) ? -1 : 1;
+  return (x->s_dst > y->s_dst) ? -1 : (x->s_dst < y->s_dst);
+}
+
+/* Fill the geometry part of pass `ps` transforming dim d from S to D. */
+static int fill_geometry(pfk_pass *ps, const arrgeo *S, const arrgeo *D, int d) {
+  INT ss[PA_MAXRNK], sd[PA_MAXRNK];
+  ps->src_elems = geo_strides(S, ss);
+  ps->dst_elems = geo_strides(D, sd);
+  ps->sin = ss[d];
+  ps->sout = sd[d];
+  bdim b[PA_MAXRNK + 1];
+  int nb = 0;
+  for (int e = 0; e < S->rnk; e++) {
+    if (e == d || S->ext[e] == 1) continue;
+    b[nb].ext = S->ext[e]; b[nb].s_src = ss[e]; b[nb].s_dst = sd[e]; nb++;
+  }
+  if (S->h > 1) { b[nb].ext = S->h; b[nb].s_src = 1; b[nb].s_dst = 1; nb++; }
+  qsort(b, (size_t)nb, sizeof(bdim), cmp_bdim);
+  /* merge dims that are contiguous with their inner neighbour on both sides */
+  int m = 0;
+  for (int i = 0; i < nb; i++) {
+    if (m > 0) {
+      bdim *o = &b[m - 1];
+      if (o->s_src == b[i].ext * b[i].s_src && o->s_dst == b[i].ext * b[i].s_dst) {
+        o->ext *= b[i].ext; o->s_src = b[i].s_src; o->s_dst = b[i].s_dst;
+        continue;
+      }
+    }
+    b[m++] = b[i];
+  }
+  nb = m;
+  int tile = -1;
+  if (ps->sin != 1)
+    for (int i = 0; i < nb; i++) if (b[i].s_src == 1) { tile = i; break; }
+  if (tile < 0 && ps->sout != 1)
+    for (int i = 0; i < nb; i++) if (b[i].s_dst == 1) { tile = i; break; }
+  if (tile < 0 && nb > 0) tile = nb - 1;   /* innermost remaining dim */
+  if (tile >= 0) {
+    ps->text = b[tile].ext; ps->tsin = b[tile].s_src; ps->tsout = b[tile].s_dst;
+    for (int i = tile; i < nb - 1; i++) b[i] = b[i + 1];
+    nb--;
+  } else {
+    ps->text = 1; ps->tsin = 0; ps->tsout = 0;
+  }
+  if (nb > PFK_MAXB) { pa_error("too many batch dimensions (%d)", nb); return 1; }
+  ps->nb = nb;
+  /* innermost batch dim first, for the kernel's div/mod decode */
+  for (int i = 0; i < nb; i++) {
+    ps->bext[i] = b[nb - 1 - i].ext;
+    ps->bsin[i] = b[nb - 1 - i].s_src;
+    ps->bsout[i] = b[nb - 1 - i].s_dst;
+  }
+  return 0;
+}
+
+/* highest element offset a pass touches on each side must fit the buffer */
+static int check_bounds(const pfk_pass *ps) {
+  int64_t ms = (ps->nin - 1) * ps->sin + (ps->text - 1) * ps->tsin;
+  int64_t md = (ps->nout - 1) * ps->sout + (ps->text - 1) * ps->tsout;
+  for (int i = 0; i < ps->nb; i++) {
+    ms += (ps->bext[i] - 1) * ps->bsin[i];
+    md += (ps->bext[i] - 1) * ps->bsout[i];
+  }
+  if (ps->sin < 0 || ps->sout < 0 || ms >= ps->src_elems || md >= ps->dst_elems) {
+    pa_error("internal: pass geometry exceeds its buffers (%lld/%lld, %lld/%lld)",
+             (long long)ms, (long long)ps->src_elems, (long long)md, (long long)ps->dst_elems);
+    return 1;
+  }
+  return 0;
+}
+
+static int64_t geo_elems(const arrgeo *g) {
+  INT s[PA_MAXRNK];
+  return geo_strides(g, s);
+}
+
+static void free_plan_memory(pa_plan *p) {
+  for (int i = 0; i < p->ntw; i++) pfk_free(p->tw_bufs[i]);
+  p->ntw = 0;
+  for (int i = 0; i < 2; i++) { pfk_free(p->scratch[i]); p->scratch[i] = NULL; }
+}
+
+pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const INT *no,
+                        INT howmany, const INT *iblock, const INT *oblock, void *in, void *out,
+                        MPI_Comm comm, int sign, const int *skip_trafos, unsigned tf,
+                        unsigned pfft_flags) {
+  const unsigned transp = pfft_flags & (PFFT_TRANSPOSED_IN | PFFT_TRANSPOSED_OUT);
+  pa_mesh mesh;
+
+  /* validation: kernel/partrafo.c:337-373 */
+  if (pfft_flags & (PFFT_SHIFTED_IN | PFFT_SHIFTED_OUT))
+    for (int t = 0; t < rnk_n; t++)
+      if (n[t] % 2) return NULL;
+  if ((transp & PFFT_TRANSPOSED_OUT) && (transp & PFFT_TRANSPOSED_IN)) return NULL;
+  if ((tf & TF_R2C) && (transp & PFFT_TRANSPOSED_IN)) return NULL;
+  if ((tf & TF_C2R) && (transp & PFFT_TRANSPOSED_OUT)) return NULL;
+  if (pa_mesh_of(comm, &mesh)) return NULL;
+  const int r = mesh.rnk_pm;
+  if (rnk_n < r) return NULL;
+  if (rnk_n == r && rnk_n != 3) return NULL;
+
+  /* limits of this implementation */
+  if (rnk_n > PA_MAXRNK || rnk_n < 1) { pa_error("rank %d not supported", rnk_n); return NULL; }
+  if (rnk_n == 3 && r == 3) {
+    pa_error("3-d process meshes (3d-to-2d remap) are not supported on one GPU");
+    return NULL;
+  }
+  if (tf & TF_R2R) { pa_error("r2r transforms are not supported"); return NULL; }
+  if (mesh.size != 1) { pa_error("process mesh must have one process"); return NULL; }
+  if (howmany < 1) return NULL;
+  for (int t = 0; t < rnk_n; t++) {
+    if (n[t] < 1 || ni[t] < 1 || no[t] < 1) { pa_error("sizes must be positive"); return NULL; }
+    if (ni[t] > n[t] || no[t] > n[t]) {
+      pa_error("ni and no must not exceed n (dim %d)", t);
+      return NULL;
+    }
+  }
+  if (skip_trafos && (tf & TF_RDFT) && skip_trafos[r]) {
+    pa_error("skipping the r2c/c2r dimension is not supported");
+    return NULL;
+  }
+
+  pa_plan *p = (pa_plan *)calloc(1, sizeof(pa_plan));
+  if (!p) return NULL;
+  p->prec = prec;
+  p->rnk_n = rnk_n;
+  p->howmany = howmany;
+  p->trafo_flag = tf;
+  p->pfft_flags = pfft_flags;
+  p->transp = transp;
+  p->sign = sign;
+  p->in = in;
+  p->out = out;
+  for (int t = 0; t < rnk_n; t++) {
+    p->n[t] = n[t]; p->ni[t] = ni[t]; p->no[t] = no[t];
+    p->skip[t] = skip_trafos ? skip_trafos[t] : 0;
+  }
+  pa_local_size(rnk_n, n, ni, no, howmany, iblock, oblock, &mesh, tf, pfft_flags, p->local_ni,
+                p->local_i_start, p->local_no, p->local_o_start);
+  {
+    /* a user block smaller than the data would leave part of the array on
+     * processes that do not exist */
+    INT lni[PA_MAXRNK], lno[PA_MAXRNK];
+    for (int t = 0; t < rnk_n; t++) { lni[t] = p->local_ni[t]; lno[t] = p->local_no[t]; }
+    const int last = rnk_n - 1;
+    for (int t = 0; t < rnk_n; t++) {
+      INT full_i = (t == last && (tf & TF_C2R)) ? ni[t] / 2 + 1 : ni[t];
+      INT full_o = (t == last && (tf & TF_R2C)) ? no[t] / 2 + 1 : no[t];
+      if (t == last && (tf & TF_R2C)) full_i = (tf & TF_PADDED) ? 2 * (ni[t] / 2 + 1) : ni[t];
+      if (t == last && (tf & TF_C2R)) full_o = (tf & TF_PADDED) ? 2 * (no[t] / 2 + 1) : no[t];
+      if (lni[t] != full_i || lno[t] != full_o) {
+        pa_error("user block sizes must cover the whole array on a one-process mesh");
+        free(p);
+        return NULL;
+      }
+    }
+  }
+
+  const int R = rnk_n, last = R - 1;
+  const int is_r2c = (tf & TF_R2C) != 0, is_c2r = (tf & TF_C2R) != 0;
+  const int padded = (tf & TF_PADDED) != 0;
+  p->in_esz = (prec == PFK_F64 ? 8 : 4) * (is_r2c ? 1 : 2);
+  p->out_esz = (prec == PFK_F64 ? 8 : 4) * (is_c2r ? 1 : 2);
+
+  /* user arrays */
+  p->gin.rnk = p->gout.rnk = R;
+  p->gin.h = p->gout.h = howmany;
+  if (transp & PFFT_TRANSPOSED_IN) order_transposed(R, r, p->gin.order);
+  else order_identity(R, p->gin.order);
+  if (transp & PFFT_TRANSPOSED_OUT) order_transposed(R, r, p->gout.order);
+  else order_identity(R, p->gout.order);
+  INT ext_in_c[PA_MAXRNK], ext_out_c[PA_MAXRNK];   /* extents of complex intermediates */
+  for (int t = 0; t < R; t++) {
+    p->gin.ext[t] = ni[t];
+    p->gout.ext[t] = no[t];
+    ext_in_c[t] = ni[t];
+    ext_out_c[t] = no[t];
+  }
+  if (is_r2c) {
+    p->gin.is_real = 1;
+    if (padded) p->gin.phys_last = 2 * (ni[last] / 2 + 1);
+    p->gout.ext[last] = no[last] / 2 + 1;
+    ext_out_c[last] = no[last] / 2 + 1;
+  }
+  if (is_c2r) {
+    p->gout.is_real = 1;
+    if (padded) p->gout.phys_last = 2 * (no[last] / 2 + 1);
+    p->gin.ext[last] = ni[last] / 2 + 1;
+    ext_in_c[last] = ni[last] / 2 + 1;
+  }
+  p->in_elems = geo_elems(&p->gin);
+  p->out_elems = geo_elems(&p->gout);
+
+  /* pass order */
+  int ord[PA_MAXRNK];
+  const int first0 = is_c2r || (!is_r2c && (transp & PFFT_TRANSPOSED_IN));
+  for (int k = 0; k < R; k++) ord[k] = first0 ? k : R - 1 - k;
+
+  int done[PA_MAXRNK] = {0};
+  arrgeo cur = p->gin, nxt;
+  int64_t scratch_elems[2] = {0, 0};
+  p->npass = R;
+  int src_role = ROLE_IN;
+  arrgeo prev_inter;
+  int have_prev = 0;
+  for (int k = 0; k < R; k++) {
+    const int d = ord[k];
+    pfk_pass *ps = &p->pass[k];
+    memset(ps, 0, sizeof(*ps));
+    done[d] = 1;
+    if (k == R - 1) {
+      nxt = p->gout;
+    } else {
+      memset(&nxt, 0, sizeof(nxt));
+      nxt.rnk = R;
+      nxt.h = howmany;
+      order_identity(R, nxt.order);
+      for (int t = 0; t < R; t++) nxt.ext[t] = done[t] ? ext_out_c[t] : ext_in_c[t];
+    }
+    const int skipped = p->skip[d < r ? d : r] != 0;
+    int kind = PFK_C2C;
+    if (d == last && is_r2c) kind = PFK_R2C;
+    else if (d == last && is_c2r) kind = PFK_C2R;
+    else if (skipped) kind = PFK_COPY;
+    ps->kind = kind;
+    ps->prec = prec;
+    ps->sign = sign;
+    ps->n = n[d];
+    ps->nf = ((kind == PFK_R2C || kind == PFK_C2R) && n[d] % 2 == 0) ? n[d] / 2 : n[d];
+    /* embed / trunc offsets: kernel/ousample.c:241-322 */
+    const int sh_in = (pfft_flags & PFFT_SHIFTED_IN) != 0;
+    const int sh_out = (pfft_flags & PFFT_SHIFTED_OUT) != 0;
+    ps->nin = ni[d];
+    ps->zin = sh_in ? (n[d] - ni[d]) / 2 : 0;
+    ps->nout = no[d];
+    ps->zout = sh_out ? (n[d] - no[d]) / 2 : 0;
+    if (kind == PFK_R2C) {
+      ps->nout = no[d] / 2 + 1;
+      ps->zout = (n[d] / 2 + 1) - (no[d] / 2 + 1);   /* keeps the upper end */
+    }
+    if (kind == PFK_C2R) {
+      ps->nin = ni[d] / 2 + 1;
+      ps->zin = (n[d] / 2 + 1) - (ni[d] / 2 + 1);    /* data at the upper end */
+    }
+    /* +-1 modulations: api/api-basic.c:1186-1285 */
+    ps->imod.on = (pfft_flags & PFFT_SHIFTED_OUT) && !p->skip[d];
+    ps->imod.start = p->local_i_start[d];
+    ps->imod.half = ni[d] / 2;
+    ps->imod.extra = (sh_in && ((n[d] / 2) % 2)) ? -1 : 1;
+    ps->omod.on = (pfft_flags & PFFT_SHIFTED_IN) && !p->skip[d];
+    ps->omod.start = p->local_o_start[d];
+    ps->omod.half = no[d] / 2;
+    ps->omod.extra = 1;
+    if (kind != PFK_COPY) {
+      ps->nstages = factorize(ps->nf, ps->radix);
+      if (ps->nstages < 0) { pa_error("cannot factor length %lld", (long long)ps->nf); goto fail; }
+      ps->tw = make_twiddles(p, ps->nf, ps->nf, sign);
+      p->tw_adj[k] = make_twiddles(p, ps->nf, ps->nf, -sign);
+      if (!ps->tw || !p->tw_adj[k]) goto fail;
+      if ((kind == PFK_R2C || kind == PFK_C2R) && n[d] % 2 == 0) {
+        ps->rtw = make_twiddles(p, n[d], n[d] / 2 + 1, sign);
+        p->rtw_adj[k] = make_twiddles(p, n[d], n[d] / 2 + 1, -sign);
+        if (!ps->rtw || !p->rtw_adj[k]) goto fail;
+      }
+    }
+    if (fill_geometry(ps, &cur, &nxt, d)) goto fail;
+    if (check_bounds(ps)) goto fail;
+    /* buffers */
+    p->src_role[k] = src_role;
+    int dst_role;
+    if (k == R - 1) {
+      dst_role = ROLE_OUT;
+    } else {
+      if (have_prev && (src_role == ROLE_S0 || src_role == ROLE_S1) && same_geo(&prev_inter, &nxt))
+        dst_role = src_role;
+      else
+        dst_role = (src_role == ROLE_S0) ? ROLE_S1 : ROLE_S0;
+      const int64_t e = geo_elems(&nxt);
+      if (e > scratch_elems[dst_role - ROLE_S0]) scratch_elems[dst_role - ROLE_S0] = e;
+      prev_inter = nxt;
+      have_prev = 1;
+    }
+    p->dst_role[k] = dst_role;
+    src_role = dst_role;
+    cur = nxt;
+  }
+  {
+    const size_t cesz = (prec == PFK_F64) ? 16 : 8;
+    for (int i = 0; i < 2; i++) {
+      p->scratch_bytes[i] = (size_t)scratch_elems[i] * cesz;
+      if (p->scratch_bytes[i] && pfk_malloc(&p->scratch[i], p->scratch_bytes[i])) {
+        pa_error("device allocation of %zu bytes failed: %s", p->scratch_bytes[i],
+                 pfk_last_error());
+        goto fail;
+      }
+    }
+  }
+  return p;
+fail:
+  free_plan_memory(p);
+  free(p);
+  return NULL;
+}
+
+void pa_plan_destroy(pa_plan *p) {
+  if (!p) return;
+  free_plan_memory(p);
+  free(p);
+}
+
+void *pa_plan_in(const pa_plan *p) { return p->in; }
+void *pa_plan_out(const pa_plan *p) { return p->out; }
+
+/* Runs the passes; `adjoint` runs them in reverse with swapped roles. */
+static void run(const pa_plan *p, void *in, void *out, int adjoint) {
+  if (!p) return;
+  const size_t in_bytes = (size_t)(adjoint ? p->out_elems * p->out_esz : p->in_elems * p->in_esz);
+  const size_t out_bytes = (size_t)(adjoint ? p->in_elems * p->in_esz : p->out_elems * p->out_esz);
+  void *din = in, *dout = out, *stage_in = NULL, *stage_out = NULL;
+  const int host_in = !pfk_is_device_ptr(in);
+  const int host_out = !pfk_is_device_ptr(out);
+  if (host_in) {
+    if (pfk_malloc(&stage_in, in_bytes) || pfk_memcpy_h2d(stage_in, in, in_bytes)) {
+      pa_error("staging input failed: %s", pfk_last_error());
+      pfk_free(stage_in);
+      return;
+    }
+    din = stage_in;
+  }
+  if (host_out) {
+    if (in == out && host_in) {
+      dout = stage_in;
+      if (out_bytes > in_bytes) {
+        pa_error("in-place host arrays: output larger than input");
+        pfk_free(stage_in);
+        return;
+      }
+    } else {
+      if (pfk_malloc(&stage_out, out_bytes)) {
+        pa_error("staging output failed: %s", pfk_last_error());
+        pfk_free(stage_in);
+        return;
+      }
+      dout = stage_out;
+    }
+  }
+  void *bufs[4] = {din, dout, p->scratch[0], p->scratch[1]};
+  int err = 0;
+  for (int i = 0; i < p->npass && !err; i++) {
+    const int k = adjoint ? p->npass - 1 - i : i;
+    const pfk_pass *fw = &p->pass[k];
+    if (!adjoint) {
+      err = pfk_launch_pass(fw, bufs[p->src_role[k]], bufs[p->dst_role[k]]);
+    } else {
+      pfk_pass a = *fw;
+      a.sign = -fw->sign;
+      a.nin = fw->nout; a.zin = fw->zout;
+      a.nout = fw->nin; a.zout = fw->zin;
+      a.imod = fw->omod; a.omod = fw->imod;
+      a.sin = fw->sout; a.sout = fw->sin;
+      a.tsin = fw->tsout; a.tsout = fw->tsin;
+      for (int b = 0; b < a.nb; b++) { a.bsin[b] = fw->bsout[b]; a.bsout[b] = fw->bsin[b]; }
+      a.src_elems = fw->dst_elems; a.dst_elems = fw->src_elems;
+      if (fw->kind == PFK_R2C) { a.kind = PFK_C2R; a.hw = 1; }
+      else if (fw->kind == PFK_C2R) { a.kind = PFK_R2C; a.hw = 2; }
+      a.tw = p->tw_adj[k]; a.rtw = p->rtw_adj[k];
+      /* map roles: forward IN <-> adjoint OUT */
+      static const int swap_io[4] = {ROLE_OUT, ROLE_IN, ROLE_S0, ROLE_S1};
+      err = pfk_launch_pass(&a, bufs[swap_io[p->dst_role[k]]], bufs[swap_io[p->src_role[k]]]);
+    }
+    if (err) pa_error("pass %d launch failed: %s", k, pfk_last_error());
+  }
+  if (!err && pfk_sync()) pa_error("execution failed: %s", pfk_last_error());
+  if (host_out && !err) {
+    if (pfk_memcpy_d2h(out, dout, out_bytes)) pa_error("copy-out failed: %s", pfk_last_error());
+  }
+  pfk_free(stage_in);
+  pfk_free(stage_out);
+}
+
+void pa_plan_execute(const pa_plan *p, void *in, void *out) { run(p, in, out, 0); }
+void pa_plan_execute_adjoint(const pa_plan *p, void *in, void *out) { run(p, in, out, 1); }
+
+void pa_plans_execute_batch(const pa_plan *const *plans, int count) {
+  for (int i = 0; i < count; i++)
+    if (plans[i]) run(plans[i], plans[i]->in, plans[i]->out, 0);
+}
