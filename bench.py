@@ -1,0 +1,197 @@
+"""Benchmark of the PFFT transform pipeline on one MI355X.
+
+Workload (the headline configuration): 512^3 FP64 real-to-complex forward
+transform with PFFT_TRANSPOSED_OUT followed by the complex-to-real
+backward transform with PFFT_TRANSPOSED_IN, out-of-place, on a 1x1
+process mesh, input i.i.d. standard normal (seeded), already resident in
+HBM.  One step = one r2c + c2r pair.
+
+Prints one JSON line (rank 0, N=1): throughput in GFLOP/s counted as
+5*N*log2(N)*0.5 per r2c or c2r transform, the per-pair time, the roofline
+of the dominant kernel (algorithmic bytes / measured duration vs the
+8 TB/s HBM peak, HIP events on the library stream) and a CPU baseline
+(the oracle timed on the host cores).
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("512³ r2c+c2r ms & %HBM peak; FP64 plain vs FP32-MFMA at stated rel-L2 error")
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-cores", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=20261202)
+    return ap.parse_args()
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary (profiles/), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic_r2c_c2r_512.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except Exception:
+        return None
+
+
+def cpu_baseline(n, cores, seed):
+    from oracle import semantics as S
+    S.WORKERS = cores
+    N = n ** 3
+    fw = S.PlanSpec("r2c", [n] * 3, flags=2)
+    bw = S.PlanSpec("c2r", [n] * 3, sign=+1, flags=1)
+    x = np.random.default_rng(seed).standard_normal(N)
+    t0 = time.perf_counter()
+    k = S.execute_flat(fw, x, precision=np.float64)
+    y = S.execute_flat(bw, k, precision=np.float64)
+    t1 = time.perf_counter()
+    err = float(np.max(np.abs(y / N - x)))
+    flops = 5.0 * N * math.log2(N)
+    return {
+        "value": flops / (t1 - t0) / 1e9,
+        "unit": "GFLOP/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": ("one full %d^3 FP64 r2c(TRANSPOSED_OUT)+c2r(TRANSPOSED_IN) pair through the "
+                   "oracle (numpy + scipy.fft pocketfft, %d worker threads); %.2f s; "
+                   "roundtrip max err %.1e; the reference PFFT/FFTW-MPI is not installed "
+                   "(no network), so this is the labelled stand-in" % (n, cores, t1 - t0, err)),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no sharding, no collectives); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import pfft_amd as P
+    L = P.lib()
+    L.pfft_amd_time_passes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]
+    L.pfft_amd_plan_npasses.argtypes = [ctypes.c_void_p]
+    L.pfft_amd_copy_bandwidth.restype = ctypes.c_double
+    L.pfft_amd_copy_bandwidth.argtypes = [ctypes.c_size_t, ctypes.c_int]
+    L.pfft_amd_describe_plan.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+
+    n = args.n
+    N = n ** 3
+    rc, comm = P.create_procmesh([1, 1])
+    mem = P.local_size("r2c", [n] * 3, flags=P.TRANSPOSED_OUT, comm=comm)[0]
+    x = np.random.default_rng(args.seed).standard_normal(N)
+    dx = P.DeviceBuffer(N * 8)
+    dk = P.DeviceBuffer(mem * 16)
+    dy = P.DeviceBuffer(2 * mem * 8)
+    fw = P.Plan("r2c", [n] * 3, dx.ptr, dk.ptr, flags=P.TRANSPOSED_OUT, comm=comm)
+    bw = P.Plan("c2r", [n] * 3, dk.ptr, dy.ptr, sign=+1, flags=P.TRANSPOSED_IN, comm=comm)
+    assert fw.ok and bw.ok
+    dx.upload(x)
+
+    for _ in range(args.warmup):
+        fw.execute()
+        bw.execute()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fw.execute()
+        bw.execute()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    flops_pair = 5.0 * N * math.log2(N)
+    gflops = flops_pair / (ms * 1e-3) / 1e9
+
+    y = dy.download(np.float64, N)
+    rt_err = float(np.max(np.abs(y / N - x)))
+
+    # per-pass durations (HIP events on the library stream)
+    passes = []
+    for name, pl, a, b in (("r2c", fw, dx, dk), ("c2r", bw, dk, dy)):
+        npass = L.pfft_amd_plan_npasses(pl.handle)
+        msv = (ctypes.c_double * npass)()
+        byv = (ctypes.c_double * npass)()
+        assert L.pfft_amd_time_passes(pl.handle, a.ptr, b.ptr, 10, msv, byv) == 0
+        desc = ctypes.create_string_buffer(4096)
+        L.pfft_amd_describe_plan(pl.handle, desc, 4096)
+        lines = desc.value.decode().strip().split("\n")
+        for k in range(npass):
+            passes.append({"plan": name, "pass": k, "ms": msv[k], "bytes": byv[k],
+                           "GBps": byv[k] / (msv[k] * 1e-3) / 1e9, "desc": lines[k]})
+    dom = max(passes, key=lambda p: p["ms"])
+    copy_gbs = L.pfft_amd_copy_bandwidth(1 << 30, 10)
+    sum_ms = sum(p["ms"] for p in passes)
+    bytes_pair = sum(p["bytes"] for p in passes)
+
+    traffic = traffic_from_profiles()
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(dom["GBps"], 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4),
+        "traffic": (traffic or {}).get("bytes_per_launch") if traffic else None,
+        "kernel": "%s pass %d (%s)" % (dom["plan"], dom["pass"], dom["desc"]),
+        "algorithmic_bytes_per_launch": dom["bytes"],
+        "avg_launch_ms": round(dom["ms"], 4),
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: i.i.d. N(0,1) real input, seed %d" % args.seed,
+        "config": {"workload": "%d^3 r2c(PFFT_TRANSPOSED_OUT) + c2r(PFFT_TRANSPOSED_IN), FP64, "
+                               "out-of-place, procmesh 1x1" % n,
+                   "n": [n, n, n], "flops_per_step": flops_pair,
+                   "alg_bytes_per_step": bytes_pair},
+        "roofline": roofline,
+        "pipeline": {
+            "sum_of_pass_ms": round(sum_ms, 4),
+            "pipeline_GBps_alg": round(bytes_pair / (sum_ms * 1e-3) / 1e9, 1),
+            "frac_of_copy_bw": round(bytes_pair / (sum_ms * 1e-3) / 1e9 / copy_gbs, 4),
+            "measured_copy_GBps": round(copy_gbs, 1),
+            "passes": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()}
+                       for p in passes],
+        },
+        "roundtrip_max_err": rt_err,
+    }
+    for o in (fw, bw):
+        o.destroy()
+    for b in (dx, dk, dy):
+        b.free()
+    if not args.no_cpu:
+        cores = args.cpu_cores or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(n, cores, args.seed)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -276,6 +276,15 @@ void pfft_execute_batch(const pfft_plan *plans, int count);
  * fill arrays from the host. 0 on success. */
 int pfft_amd_memcpy_to_device(void *dst_dev, const void *src_host, size_t bytes);
 int pfft_amd_memcpy_to_host(void *dst_host, const void *src_dev, size_t bytes);
+/* Measurement hooks: number of device passes of a plan; mean duration
+ * (ms) of each pass over `iters` runs on device arrays in/out together
+ * with the bytes each pass reads plus writes; a text description of the
+ * passes; the device's streaming-copy bandwidth in GB/s. */
+int pfft_amd_plan_npasses(const void *plan);
+int pfft_amd_time_passes(const void *plan, void *in, void *out, int iters, double *ms,
+                         double *bytes);
+int pfft_amd_describe_plan(const void *plan, char *buf, size_t len);
+double pfft_amd_copy_bandwidth(size_t bytes, int iters);
 /* Library version string. */
 const char *pfft_amd_version(void);
 /* Last error message of the device runtime (empty string if none). */

@@ -30,6 +30,10 @@ from . import layout as L
 FORWARD = -1
 BACKWARD = +1
 
+# host threads the DFTs may use (scipy.fft workers); the CPU baseline
+# sets this to the core count it reports
+WORKERS = 1
+
 
 class PlanSpec:
     """Parameters of a plan: what PX(plan_many_dft*) receives."""
@@ -260,6 +264,8 @@ def embed_axis(arr, axis, params):
     if params is None:
         return arr
     Zl, D, Zr = params
+    if Zl == 0 and Zr == 0 and D == arr.shape[axis]:
+        return arr
     a = np.take(arr, np.arange(D), axis=axis)
     pad = [(0, 0)] * arr.ndim
     pad[axis] = (Zl, Zr)
@@ -270,24 +276,26 @@ def trunc_axis(arr, axis, params):
     if params is None:
         return arr
     Zl, D, Zr = params
+    if Zl == 0 and D == arr.shape[axis]:
+        return arr
     return np.take(arr, np.arange(Zl, Zl + D), axis=axis)
 
 
 # ---------------------------------------------------------------- DFTs
 def dft_c2c(arr, axis, sign):
     if sign == FORWARD:
-        return sfft.fft(arr, axis=axis)
-    return sfft.ifft(arr, axis=axis) * arr.shape[axis]
+        return sfft.fft(arr, axis=axis, workers=WORKERS)
+    return sfft.ifft(arr, axis=axis, workers=WORKERS) * arr.shape[axis]
 
 
 def dft_r2c(arr, axis, n):
     """FFTW r2c: real input of length n, forward sign, n//2+1 outputs."""
-    return sfft.rfft(np.real(arr), n=n, axis=axis)
+    return sfft.rfft(np.real(arr), n=n, axis=axis, workers=WORKERS)
 
 
 def dft_c2r(arr, axis, n):
     """FFTW c2r: n//2+1 complex inputs, backward sign, n real outputs."""
-    return sfft.irfft(arr, n=n, axis=axis) * n
+    return sfft.irfft(arr, n=n, axis=axis, workers=WORKERS) * n
 
 
 # ---------------------------------------------------------------- execute

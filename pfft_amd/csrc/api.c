@@ -167,3 +167,14 @@ int pfft_amd_memcpy_to_host(void *dst, const void *src, size_t bytes) {
 }
 const char *pfft_amd_version(void) { return "pfft-amd 0.1 (gfx950)"; }
 const char *pfft_amd_last_error(void) { return pfk_last_error(); }
+
+int pfft_amd_plan_npasses(const void *plan) { return pa_plan_npasses((const pa_plan *)plan); }
+int pfft_amd_time_passes(const void *plan, void *in, void *out, int iters, double *ms,
+                         double *bytes) {
+  return pa_plan_time_passes((const pa_plan *)plan, in, out, iters, ms, bytes);
+}
+int pfft_amd_describe_plan(const void *plan, char *buf, size_t len) {
+  if (!plan || !buf || !len) return 1;
+  return pa_plan_describe((const pa_plan *)plan, buf, len);
+}
+double pfft_amd_copy_bandwidth(size_t bytes, int iters) { return pfk_copy_bandwidth(bytes, iters); }

@@ -62,6 +62,10 @@ void pa_plan_destroy(pa_plan *p);
 void *pa_plan_in(const pa_plan *p);
 void *pa_plan_out(const pa_plan *p);
 void pa_plans_execute_batch(const pa_plan *const *plans, int count);
+int pa_plan_time_passes(const pa_plan *p, void *in, void *out, int iters, double *ms,
+                        double *bytes);
+int pa_plan_npasses(const pa_plan *p);
+int pa_plan_describe(const pa_plan *p, char *buf, size_t len);
 
 /* ---- twiddles (twiddle.c) ---- */
 /* exp(sign*2*pi*i*m/N) for m in [0, count), rounded to double, computed in

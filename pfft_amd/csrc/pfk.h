@@ -103,6 +103,10 @@ int  pfk_event_destroy(void *ev);
 /* Complex conjugation / scaling utility kernels on device arrays. */
 int  pfk_scale(void *data, int prec, int64_t nreal, double factor);
 
+/* Streaming-copy bandwidth of the device (GB/s, read+write bytes counted)
+ * over a buffer of `bytes`, best of `iters` timed copies; <0 on error. */
+double pfk_copy_bandwidth(size_t bytes, int iters);
+
 #ifdef __cplusplus
 }
 #endif

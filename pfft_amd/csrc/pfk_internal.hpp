@@ -16,6 +16,7 @@ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
 }
 
 extern hipStream_t g_stream;
+extern int g_generic_only;   // PFFT_AMD_GENERIC_ONLY=1: route every pass to the general kernel
 extern int g_max_lds;
 void set_error(const char *what, hipError_t e);
 void set_error_msg(const char *msg);

@@ -362,6 +362,11 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
       nxt.h = howmany;
       order_identity(R, nxt.order);
       for (int t = 0; t < R; t++) nxt.ext[t] = done[t] ? ext_out_c[t] : ext_in_c[t];
+      /* rows of the intermediate arrays start on 128-byte boundaries so the
+       * 128-byte segments the pass kernels move are whole cache lines */
+      const INT align = (prec == PFK_F64) ? 8 : 16;
+      if (howmany == 1 && nxt.ext[last] >= align && nxt.ext[last] % align)
+        nxt.phys_last = (nxt.ext[last] + align - 1) / align * align;
     }
     const int skipped = p->skip[d < r ? d : r] != 0;
     int kind = PFK_C2C;
@@ -530,4 +535,64 @@ void pa_plan_execute_adjoint(const pa_plan *p, void *in, void *out) { run(p, in,
 void pa_plans_execute_batch(const pa_plan *const *plans, int count) {
   for (int i = 0; i < count; i++)
     if (plans[i]) run(plans[i], plans[i]->in, plans[i]->out, 0);
+}
+
+/* Times every pass of the plan over `iters` back-to-back executions on
+ * device arrays (HIP events on the library stream between passes).
+ * ms[k] receives the mean duration of pass k; bytes[k] the bytes pass k
+ * reads plus writes (one sweep of its source and destination). */
+int pa_plan_time_passes(const pa_plan *p, void *in, void *out, int iters, double *ms,
+                        double *bytes) {
+  if (!p || iters < 1 || !pfk_is_device_ptr(in) || !pfk_is_device_ptr(out)) return 1;
+  void *bufs[4] = {in, out, p->scratch[0], p->scratch[1]};
+  void *ev[PA_MAXRNK + 1];
+  for (int k = 0; k <= p->npass; k++)
+    if (pfk_event_create(&ev[k])) return 1;
+  for (int k = 0; k < p->npass; k++) ms[k] = 0;
+  int err = 0;
+  for (int it = 0; it < iters && !err; it++) {
+    pfk_event_record(ev[0]);
+    for (int k = 0; k < p->npass && !err; k++) {
+      err = pfk_launch_pass(&p->pass[k], bufs[p->src_role[k]], bufs[p->dst_role[k]]);
+      pfk_event_record(ev[k + 1]);
+    }
+    for (int k = 0; k < p->npass && !err; k++) {
+      float t = 0;
+      if (pfk_event_elapsed_ms(ev[k], ev[k + 1], &t)) err = 1;
+      ms[k] += t;
+    }
+  }
+  for (int k = 0; k < p->npass; k++) {
+    ms[k] /= iters;
+    const pfk_pass *ps = &p->pass[k];
+    const double rs = (p->prec == PFK_F64) ? 8.0 : 4.0;
+    const double src_sz = (ps->kind == PFK_R2C) ? rs : 2 * rs;
+    const double dst_sz = (ps->kind == PFK_C2R) ? rs : 2 * rs;
+    double nl = (double)ps->text;
+    for (int b = 0; b < ps->nb; b++) nl *= (double)ps->bext[b];
+    bytes[k] = nl * ((double)ps->nin * src_sz + (double)ps->nout * dst_sz);
+  }
+  for (int k = 0; k <= p->npass; k++) pfk_event_destroy(ev[k]);
+  return err;
+}
+
+int pa_plan_npasses(const pa_plan *p) { return p ? p->npass : 0; }
+
+/* One line per pass: kind, length, strides, batch and tile geometry. */
+int pa_plan_describe(const pa_plan *p, char *buf, size_t len) {
+  static const char *kn[] = {"c2c", "r2c", "c2r", "copy"};
+  size_t o = 0;
+  buf[0] = 0;
+  for (int k = 0; k < p->npass && o < len; k++) {
+    const pfk_pass *ps = &p->pass[k];
+    o += (size_t)snprintf(buf + o, len - o,
+                          "pass %d: %s n=%lld nf=%lld nin=%lld@%lld nout=%lld@%lld sin=%lld "
+                          "sout=%lld tile=%lld(%lld,%lld) nb=%d roles=%d->%d\n",
+                          k, kn[ps->kind], (long long)ps->n, (long long)ps->nf,
+                          (long long)ps->nin, (long long)ps->zin, (long long)ps->nout,
+                          (long long)ps->zout, (long long)ps->sin, (long long)ps->sout,
+                          (long long)ps->text, (long long)ps->tsin, (long long)ps->tsout, ps->nb,
+                          p->src_role[k], p->dst_role[k]);
+  }
+  return 0;
 }

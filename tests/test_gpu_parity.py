@@ -170,3 +170,39 @@ def test_roundtrip_512_r2c_c2r():
         o.destroy()
     for b in (dx, dk, dy):
         b.free()
+
+
+# ---- power-of-two shapes: these run the register-resident kernels ----
+@pytest.mark.parametrize("flags", [0, TO, TI])
+@pytest.mark.parametrize("rnk_pm", [1, 2])
+@pytest.mark.parametrize("n", [[64, 128, 256], [512, 64, 128], [128, 1024, 64]])
+def test_pow2_c2c_layouts(n, flags, rnk_pm):
+    check("c2c", n, flags=flags, rnk_pm=rnk_pm)
+    check("c2c", n, flags=flags, rnk_pm=rnk_pm, sign=+1)
+
+
+@pytest.mark.parametrize("kind,flags", [("r2c", 0), ("r2c", TO), ("r2c", TO | PAD),
+                                        ("c2r", 0), ("c2r", TI), ("c2r", TI | PAD)])
+@pytest.mark.parametrize("n", [[64, 32, 256], [128, 128, 512], [32, 64, 2048]])
+def test_pow2_real(kind, flags, n):
+    sign = -1 if kind == "r2c" else +1
+    check(kind, n, sign=sign, flags=flags)
+    check(kind, n, sign=-sign, flags=flags)
+
+
+@pytest.mark.parametrize("kind", ["c2c", "r2c", "c2r"])
+@pytest.mark.parametrize("flags", [SI | SO, SI, SO, SI | SO | TO, SI | SO | TI])
+def test_pow2_shift_ousam(kind, flags):
+    # configuration-5 geometry scaled down: ni=64^3 embedded in n=128^3
+    if (kind == "r2c" and flags & TI) or (kind == "c2r" and flags & TO):
+        pytest.skip("invalid combination (NULL plan)")
+    sign = +1 if kind == "c2r" else -1
+    check(kind, [128, 128, 128], ni=[64, 64, 64], no=[128, 128, 128], sign=sign, flags=flags)
+    check(kind, [128, 128, 128], ni=[128, 128, 128], no=[64, 64, 64], sign=sign, flags=flags)
+
+
+@pytest.mark.parametrize("kind,flags", [("c2c", TO), ("r2c", TO), ("c2r", TI)])
+def test_pow2_howmany_fp32(kind, flags):
+    sign = +1 if kind == "c2r" else -1
+    check(kind, [64, 128, 64], howmany=2, sign=sign, flags=flags)
+    check(kind, [64, 128, 64], sign=sign, flags=flags, prec="f32")
