@@ -1,0 +1,70 @@
+"""The C-ABI library loads and exports every function include/pfft.h
+declares (CPU only: no compute calls)."""
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+import pfft_amd as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_loads():
+    assert os.path.exists(P.LIB_PATH)
+    L = P.lib()
+    assert L.pfft_amd_version().decode().startswith("pfft-amd")
+
+
+def test_every_declared_symbol_is_exported():
+    names = P.exported_symbols()
+    assert len(names) > 140
+    L = P.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_reference_entry_points_present():
+    # the pfft.h entry points the reference's tests and users bind
+    L = P.lib()
+    for pre in ("pfft_", "pfftf_"):
+        for nm in ("init", "cleanup", "create_procmesh", "create_procmesh_1d",
+                   "create_procmesh_2d", "local_size_dft_3d", "local_size_dft_r2c_3d",
+                   "local_size_dft_c2r_3d", "local_size_many_dft", "plan_dft_3d",
+                   "plan_dft_r2c_3d", "plan_dft_c2r_3d", "plan_many_dft",
+                   "plan_many_dft_r2c", "plan_many_dft_c2r", "plan_many_dft_skipped",
+                   "execute", "execute_dft", "execute_dft_r2c", "execute_dft_c2r",
+                   "destroy_plan", "alloc_real", "alloc_complex", "malloc", "free",
+                   "init_input_complex_3d", "check_output_complex_3d",
+                   "init_input_real_3d", "check_output_real_3d", "local_block_dft_3d"):
+            assert hasattr(L, pre + nm), pre + nm
+
+
+def test_header_compiles_as_c_and_links(tmp_path):
+    """A C program written against include/pfft.h (reference API style)
+    compiles and links against libpfft.so."""
+    src = r'''
+#include <stdio.h>
+#include "pfft.h"
+int main(void) {
+  ptrdiff_t n[3] = {29, 27, 31}, lni[3], lis[3], lno[3], los[3];
+  MPI_Comm cart;
+  if (pfft_create_procmesh_2d(MPI_COMM_WORLD, 1, 1, &cart)) return 1;
+  ptrdiff_t m = pfft_local_size_dft_3d(n, cart, PFFT_TRANSPOSED_NONE, lni, lis, lno, los);
+  printf("%td %td %td %td\n", m, lni[0], lno[2], los[1]);
+  return m == 29 * 27 * 31 ? 0 : 2;
+}
+'''
+    c = str(tmp_path / "abi_probe.c")
+    exe = str(tmp_path / "abi_probe")
+    with open(c, "w") as f:
+        f.write(src)
+    libdir = os.path.dirname(P.LIB_PATH)
+    r = subprocess.run(["gcc", "-std=c99", "-I" + os.path.join(ROOT, "include"), c, "-o", exe,
+                        "-L" + libdir, "-lpfft", "-Wl,-rpath," + libdir],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    assert r.stdout.split() == ["24273", "29", "31", "0"]
