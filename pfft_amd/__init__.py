@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libpfft.so")
+LIB_PATH = os.environ.get("PFFT_AMD_LIB") or os.path.join(HERE, "lib", "libpfft.so")
 
 # api/pfft.h:528-545
 TRANSPOSED_NONE = 0

@@ -27,7 +27,12 @@
 // O_k = (X_k - conj X_{h-k}) W^k before the transform and stores
 // x_2p = Re z_p, x_2p+1 = Im z_p.  The mirror partner Z_{h-k} lives in
 // another thread, so both use one extra LDS exchange.
+#include <stdlib.h>
 #include "pfk_internal.hpp"
+
+#ifndef PFK_NONTEMPORAL
+#define PFK_NONTEMPORAL 1
+#endif
 
 namespace pfk {
 
@@ -82,6 +87,31 @@ __device__ __forceinline__ void dft8(C2<T> *v) {
   v[6] = csub(e2, t2);
   v[3] = cadd(e3, t3);
   v[7] = csub(e3, t3);
+}
+
+// Streaming global accesses: every element is read once and written once
+// per pass, so loads and stores bypass cache retention (nontemporal).
+template <typename T>
+__device__ __forceinline__ C2<T> ld_stream(const C2<T> *p) {
+#if PFK_NONTEMPORAL
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(p));
+  return C2<T>{v.x, v.y};
+#else
+  return *p;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(C2<T> *p, C2<T> v) {
+#if PFK_NONTEMPORAL
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  v2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<v2 *>(p));
+#else
+  *p = v;
+#endif
 }
 
 __device__ __forceinline__ int msign(const pfk_mod &m, int64_t j) {
@@ -177,9 +207,20 @@ struct TileCtx {
   int nl;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  With remap on, XCD x runs a contiguous range of tiles in order, so
+// tiles that share cache lines at their edges run on the same L2 close in
+// time.  The map is a bijection on [0, nblk) for any nblk.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nblk) {
+  const int64_t q = nblk >> 3, r = nblk & 7;
+  const int64_t x = b & 7, j = b >> 3;
+  return x * q + (x < r ? x : r) + j;
+}
+
 __device__ __forceinline__ TileCtx tile_of(const pfk_pass &P, int C, int64_t tile_base) {
   const int64_t ntt = (P.text + C - 1) / C;
-  const int64_t tileid = tile_base + blockIdx.x;
+  const int64_t tileid =
+      tile_base + (P.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x);
   const int64_t tt = tileid % ntt;
   int64_t bflat = tileid / ntt;
   const int64_t c0 = tt * C;
@@ -218,7 +259,7 @@ k_c2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
     const int j = t + TT * m - zin;
     C2<T> x{0, 0};
     if (valid && j >= 0 && j < nin) {
-      x = s[(int64_t)j * P.sin];
+      x = ld_stream(s + (int64_t)j * P.sin);
       const T f = (T)msign(P.imod, j);
       x.x *= f;
       x.y *= f;
@@ -232,7 +273,7 @@ k_c2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
       const int k = t + TT * m - zout;
       if (k >= 0 && k < nout) {
         const T f = (T)msign(P.omod, k);
-        d[(int64_t)k * P.sout] = C2<T>{v[m].x * f, v[m].y * f};
+        st_stream(d + (int64_t)k * P.sout, C2<T>{v[m].x * f, v[m].y * f});
       }
     }
   }
@@ -262,7 +303,7 @@ k_r2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
     const int p = t + TT * m;
     C2<T> x{0, 0};
     if (valid) {
-      x = s[p];
+      x = ld_stream(s + p);
       if (P.imod.on) {
         x.x *= (T)msign(P.imod, 2 * p);
         x.y *= (T)msign(P.imod, 2 * p + 1);
@@ -290,7 +331,7 @@ k_r2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
         T f = (T)msign(P.omod, ko);
         if (P.hw == 2 && k != 0) f *= (T)2;
         const C2<T> WO = cmul(O, rtw[k]);
-        d[(int64_t)ko * P.sout] = C2<T>{(E.x + WO.x) * f, (E.y + WO.y) * f};
+        st_stream(d + (int64_t)ko * P.sout, C2<T>{(E.x + WO.x) * f, (E.y + WO.y) * f});
       }
       if (k == 0) {  // Nyquist X_h = E_0 - O_0 (W^h = -1)
         const int kn = H - zout;
@@ -330,7 +371,7 @@ k_c2r_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
     const int j = k - zin;
     C2<T> x{0, 0};
     if (valid && j >= 0 && j < nin) {
-      x = s[(int64_t)j * P.sin];
+      x = ld_stream(s + (int64_t)j * P.sin);
       T f = (T)msign(P.imod, j);
       if (P.hw == 1 && k != 0) f *= (T)0.5;
       x.x *= f;
@@ -377,7 +418,7 @@ k_c2r_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
         x.x *= (T)msign(P.omod, 2 * p);
         x.y *= (T)msign(P.omod, 2 * p + 1);
       }
-      d[p] = x;
+      st_stream(d + p, x);
     }
   }
 }
@@ -416,13 +457,18 @@ static int launch_nf(const pfk_pass *p, const void *src, void *dst) {
   return 0;
 }
 
+static int g_cols512 = 8;
+
 template <typename T>
 static int launch_pow2(const pfk_pass *p, const void *src, void *dst) {
   switch (p->nf) {
     case 64: return launch_nf<T, 64, 8>(p, src, dst);
     case 128: return launch_nf<T, 128, 8>(p, src, dst);
     case 256: return launch_nf<T, 256, 8>(p, src, dst);
-    case 512: return launch_nf<T, 512, 8>(p, src, dst);
+    case 512:
+      if (p->kind == PFK_C2C && g_cols512 == 4) return launch_nf<T, 512, 4>(p, src, dst);
+      if (p->kind == PFK_C2C && g_cols512 == 16) return launch_nf<T, 512, 16>(p, src, dst);
+      return launch_nf<T, 512, 8>(p, src, dst);
     case 1024: return launch_nf<T, 1024, 4>(p, src, dst);
     case 2048: return launch_nf<T, 2048, 2>(p, src, dst);
     default: return 1;
@@ -431,6 +477,14 @@ static int launch_pow2(const pfk_pass *p, const void *src, void *dst) {
 
 int launch_fast(const pfk_pass *p, const void *src, void *dst) {
   if (g_generic_only) return 1;
+  {
+    static int init = 0;
+    if (!init) {
+      const char *e = getenv("PFFT_AMD_COLS512");
+      if (e) g_cols512 = atoi(e);
+      init = 1;
+    }
+  }
   if (p->kind == PFK_COPY) return 1;
   const int64_t nf = p->nf;
   if (nf < 64 || nf > 2048 || (nf & (nf - 1))) return 1;

@@ -342,6 +342,10 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
   const int first0 = is_c2r || (!is_r2c && (transp & PFFT_TRANSPOSED_IN));
   for (int k = 0; k < R; k++) ord[k] = first0 ? k : R - 1 - k;
 
+  const char *ev = getenv("PFFT_AMD_PAD_ROWS");
+  const int pad_rows = ev ? (ev[0] == '1') : 1;
+  ev = getenv("PFFT_AMD_XCD_REMAP");
+  const int xcd_remap = ev ? (ev[0] == '1') : 0;
   int done[PA_MAXRNK] = {0};
   arrgeo cur = p->gin, nxt;
   int64_t scratch_elems[2] = {0, 0};
@@ -365,7 +369,7 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
       /* rows of the intermediate arrays start on 128-byte boundaries so the
        * 128-byte segments the pass kernels move are whole cache lines */
       const INT align = (prec == PFK_F64) ? 8 : 16;
-      if (howmany == 1 && nxt.ext[last] >= align && nxt.ext[last] % align)
+      if (pad_rows && howmany == 1 && nxt.ext[last] >= align && nxt.ext[last] % align)
         nxt.phys_last = (nxt.ext[last] + align - 1) / align * align;
     }
     const int skipped = p->skip[d < r ? d : r] != 0;
@@ -374,6 +378,7 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     else if (d == last && is_c2r) kind = PFK_C2R;
     else if (skipped) kind = PFK_COPY;
     ps->kind = kind;
+    ps->xcd_remap = xcd_remap;
     ps->prec = prec;
     ps->sign = sign;
     ps->n = n[d];
