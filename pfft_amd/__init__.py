@@ -291,3 +291,12 @@ def exported_symbols():
     for fn in re.findall(r"\b(pfftf?_\w+|pfft_amd_\w+)\s*\(", rest):
         names.add(fn)
     return sorted(names)
+
+
+def execute_batch(plans):
+    """pfft_execute_batch / pfftf_execute_batch over Plan objects of one
+    precision (one persistent launch, replayed from a captured graph)."""
+    if not plans:
+        return
+    arr = (VP * len(plans))(*[p.handle for p in plans])
+    getattr(lib(), plans[0].P + "execute_batch")(arr, len(plans))

@@ -1,0 +1,268 @@
+// pass_tile.hpp -- the general pass computation for one tile of lines,
+// shared by the per-pass launch (kernels_generic.hip) and the persistent
+// batch kernel (kernels_batch.hip).
+#pragma once
+#include "pfk_internal.hpp"
+
+namespace pfk {
+
+__device__ __forceinline__ int mod_sign(const pfk_mod &m, int64_t j) {
+  if (!m.on) return 1;
+  const int64_t t = j + m.start;
+  if (t < m.half) return ((t & 1) ? -1 : 1) * m.extra;
+  return 1;
+}
+
+// One tile (C lines) of a pass: the whole pass computation for the lines
+// starting at tile `tileid`.  Used by the per-pass launch below and by the
+// persistent batch kernel (kernels_batch.hip).  Ends with the stores; the
+// caller synchronises before reusing the LDS buffer.
+template <typename T, int KIND, int EPT>
+__device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const int pitch,
+                                          const int64_t tileid, const T *__restrict__ src,
+                                          T *__restrict__ dst, unsigned char *smem_raw) {
+  using C2 = cpx<T>;
+  C2 *buf = reinterpret_cast<C2 *>(smem_raw);
+  T *rbuf = reinterpret_cast<T *>(smem_raw);
+  const int tid = threadIdx.x, NT = blockDim.x;
+
+  const int64_t ntt = (P.text + C - 1) / C;
+  const int64_t tt = tileid % ntt;
+  int64_t bflat = tileid / ntt;
+  const int64_t c0 = tt * C;
+  const int nl = (int)((P.text - c0) < C ? (P.text - c0) : C);
+  int64_t sb = c0 * P.tsin, db = c0 * P.tsout;
+  for (int i = 0; i < P.nb; ++i) {
+    const int64_t e = P.bext[i];
+    const int64_t d = bflat % e;
+    bflat /= e;
+    sb += d * P.bsin[i];
+    db += d * P.bsout[i];
+  }
+
+  const int n = (int)P.n, nf = (int)P.nf, nin = (int)P.nin, zin = (int)P.zin;
+  const int nout = (int)P.nout, zout = (int)P.zout;
+  const bool ld_tile_fast = (P.tsin == 1 && P.sin != 1);
+  const bool st_tile_fast = (P.tsout == 1 && P.sout != 1);
+
+  // ---------------- load (embed + input modulation) ----------------
+  if (KIND == PFK_C2C || KIND == PFK_COPY) {
+    const C2 *s = reinterpret_cast<const C2 *>(src);
+    if (nin < n)
+      for (int idx = tid; idx < nl * n; idx += NT) {
+        const int cc = idx / n, p = idx - cc * n;
+        if (p < zin || p >= zin + nin) buf[cc * pitch + p] = C2{0, 0};
+      }
+    for (int idx = tid; idx < nl * nin; idx += NT) {
+      int cc, j;
+      if (ld_tile_fast) { cc = idx % nl; j = idx / nl; }
+      else { j = idx % nin; cc = idx / nin; }
+      C2 v = s[sb + cc * P.tsin + (int64_t)j * P.sin];
+      const T f = (T)mod_sign(P.imod, j);
+      buf[cc * pitch + zin + j] = C2{v.x * f, v.y * f};
+    }
+  } else if (KIND == PFK_R2C) {
+    if ((n & 1) == 0) {
+      // real slot p of line cc is rbuf[2*cc*pitch + p]: complex slot p/2
+      for (int idx = tid; idx < nl * n; idx += NT) {
+        const int cc = idx / n, p = idx - cc * n;
+        if (p < zin || p >= zin + nin) rbuf[2 * cc * pitch + p] = (T)0;
+      }
+      for (int idx = tid; idx < nl * nin; idx += NT) {
+        int cc, j;
+        if (ld_tile_fast) { cc = idx % nl; j = idx / nl; }
+        else { j = idx % nin; cc = idx / nin; }
+        const T v = src[sb + cc * P.tsin + (int64_t)j * P.sin];
+        rbuf[2 * cc * pitch + zin + j] = v * (T)mod_sign(P.imod, j);
+      }
+    } else {
+      for (int idx = tid; idx < nl * n; idx += NT) {
+        const int cc = idx / n, p = idx - cc * n;
+        if (p < zin || p >= zin + nin) buf[cc * pitch + p] = C2{0, 0};
+      }
+      for (int idx = tid; idx < nl * nin; idx += NT) {
+        int cc, j;
+        if (ld_tile_fast) { cc = idx % nl; j = idx / nl; }
+        else { j = idx % nin; cc = idx / nin; }
+        const T v = src[sb + cc * P.tsin + (int64_t)j * P.sin];
+        buf[cc * pitch + zin + j] = C2{v * (T)mod_sign(P.imod, j), (T)0};
+      }
+    }
+  } else {  // PFK_C2R
+    const C2 *s = reinterpret_cast<const C2 *>(src);
+    if ((n & 1) == 0) {
+      const int h = n / 2;
+      if (nin < h + 1)
+        for (int idx = tid; idx < nl * (h + 1); idx += NT) {
+          const int cc = idx / (h + 1), p = idx - cc * (h + 1);
+          if (p < zin || p >= zin + nin) buf[cc * pitch + p] = C2{0, 0};
+        }
+      for (int idx = tid; idx < nl * nin; idx += NT) {
+        int cc, j;
+        if (ld_tile_fast) { cc = idx % nl; j = idx / nl; }
+        else { j = idx % nin; cc = idx / nin; }
+        C2 v = s[sb + cc * P.tsin + (int64_t)j * P.sin];
+        T f = (T)mod_sign(P.imod, j);
+        const int p = zin + j;
+        if (P.hw == 1 && p != 0 && p != h) f *= (T)0.5;
+        buf[cc * pitch + p] = C2{v.x * f, v.y * f};
+      }
+    } else {
+      for (int idx = tid; idx < nl * n; idx += NT) {
+        const int cc = idx / n, p = idx - cc * n;
+        buf[cc * pitch + p] = C2{0, 0};
+      }
+      __syncthreads();
+      for (int idx = tid; idx < nl * nin; idx += NT) {
+        int cc, j;
+        if (ld_tile_fast) { cc = idx % nl; j = idx / nl; }
+        else { j = idx % nin; cc = idx / nin; }
+        C2 v = s[sb + cc * P.tsin + (int64_t)j * P.sin];
+        T f = (T)mod_sign(P.imod, j);
+        const int p = zin + j;
+        if (P.hw == 1 && p != 0) f *= (T)0.5;
+        if (p == 0) {
+          buf[cc * pitch] = C2{v.x * f, (T)0};
+        } else {
+          buf[cc * pitch + p] = C2{v.x * f, v.y * f};
+          buf[cc * pitch + n - p] = C2{v.x * f, -v.y * f};
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  const C2 *tw = reinterpret_cast<const C2 *>(P.tw);
+  const C2 *rtw = reinterpret_cast<const C2 *>(P.rtw);
+
+  // ------------- c2r (even n): pack the half spectrum into Z ----------
+  if (KIND == PFK_C2R && (n & 1) == 0) {
+    const int h = n / 2, hp = h / 2 + 1;
+    for (int idx = tid; idx < nl * hp; idx += NT) {
+      const int cc = idx / hp, k = idx - cc * hp;
+      C2 *L = buf + cc * pitch;
+      if (k == 0) {
+        const T x0 = L[0].x, xh = L[h].x;
+        L[0] = C2{x0 + xh, x0 - xh};
+      } else {
+        const int k2 = h - k;
+        const C2 a = L[k], b = L[k2];
+        // E_k = X_k + conj(X_{h-k}),  O_k = (X_k - conj(X_{h-k})) W^k,  Z_k = E_k + i O_k
+        {
+          const C2 E{a.x + b.x, a.y - b.y};
+          const C2 O = cmul(C2{a.x - b.x, a.y + b.y}, rtw[k]);
+          L[k] = C2{E.x - O.y, E.y + O.x};
+        }
+        if (k2 != k) {
+          const C2 E{b.x + a.x, b.y - a.y};
+          const C2 O = cmul(C2{b.x - a.x, b.y + a.y}, rtw[k2]);
+          L[k2] = C2{E.x - O.y, E.y + O.x};
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---------------- FFT stages (output-centric Stockham) ----------------
+  if (KIND != PFK_COPY) {
+    int L = 1;
+    for (int s = 0; s < P.nstages; ++s) {
+      const int R = P.radix[s];
+      const int LR = L * R, M = nf / LR, NR = nf / R;
+      C2 res[EPT];
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int og = tid + e * NT;
+        if (og < nl * nf) {
+          const int cc = og / nf, o = og - cc * nf;
+          const int k = o % L, q = (o / L) % R, t = (o / LR) * L + k;
+          const C2 *Lp = buf + cc * pitch + t;
+          C2 acc = Lp[0];
+          const int kq = k + q * L;
+          int ex = 0;
+          for (int r = 1; r < R; ++r) {
+            ex += kq;
+            if (ex >= LR) ex -= LR;
+            const C2 w = tw[ex * M];
+            const C2 v = Lp[r * NR];
+            acc.x += v.x * w.x - v.y * w.y;
+            acc.y += v.x * w.y + v.y * w.x;
+          }
+          res[e] = acc;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int og = tid + e * NT;
+        if (og < nl * nf) {
+          const int cc = og / nf, o = og - cc * nf;
+          buf[cc * pitch + o] = res[e];
+        }
+      }
+      __syncthreads();
+      L = LR;
+    }
+  }
+
+  // ---------------- store (trunc + output modulation) ----------------
+  if (KIND == PFK_C2C || KIND == PFK_COPY) {
+    C2 *d = reinterpret_cast<C2 *>(dst);
+    for (int idx = tid; idx < nl * nout; idx += NT) {
+      int cc, k;
+      if (st_tile_fast) { cc = idx % nl; k = idx / nl; }
+      else { k = idx % nout; cc = idx / nout; }
+      const C2 v = buf[cc * pitch + zout + k];
+      const T f = (T)mod_sign(P.omod, k);
+      d[db + cc * P.tsout + (int64_t)k * P.sout] = C2{v.x * f, v.y * f};
+    }
+  } else if (KIND == PFK_R2C) {
+    C2 *d = reinterpret_cast<C2 *>(dst);
+    const int h = n / 2;
+    for (int idx = tid; idx < nl * nout; idx += NT) {
+      int cc, k;
+      if (st_tile_fast) { cc = idx % nl; k = idx / nl; }
+      else { k = idx % nout; cc = idx / nout; }
+      const int kk = zout + k;
+      C2 X;
+      if ((n & 1) == 0) {
+        const C2 *L = buf + cc * pitch;
+        const C2 a = L[kk == h ? 0 : kk];
+        const C2 b = L[kk == 0 ? 0 : h - kk];
+        // E = (Z_k + conj Z_{h-k})/2 ; O = (Z_k - conj Z_{h-k})/(2i) ; X = E + W^k O
+        const C2 E{(a.x + b.x) * (T)0.5, (a.y - b.y) * (T)0.5};
+        const C2 O{(a.y + b.y) * (T)0.5, -(a.x - b.x) * (T)0.5};
+        const C2 WO = cmul(O, rtw[kk]);
+        X = C2{E.x + WO.x, E.y + WO.y};
+      } else {
+        X = buf[cc * pitch + kk];
+      }
+      T f = (T)mod_sign(P.omod, k);
+      if (P.hw == 2 && kk != 0 && !((n & 1) == 0 && kk == n / 2)) f *= (T)2;
+      d[db + cc * P.tsout + (int64_t)k * P.sout] = C2{X.x * f, X.y * f};
+    }
+  } else {  // C2R
+    for (int idx = tid; idx < nl * nout; idx += NT) {
+      int cc, k;
+      if (st_tile_fast) { cc = idx % nl; k = idx / nl; }
+      else { k = idx % nout; cc = idx / nout; }
+      const int j = zout + k;
+      T x;
+      if ((n & 1) == 0) {
+        const C2 z = buf[cc * pitch + (j >> 1)];
+        x = (j & 1) ? z.y : z.x;
+      } else {
+        x = buf[cc * pitch + j].x;
+      }
+      dst[db + cc * P.tsout + (int64_t)k * P.sout] = x * (T)mod_sign(P.omod, k);
+    }
+  }
+}
+
+
+// Tile width C and LDS row pitch (complex slots) of the general kernel for
+// pass p; returns 0 and fills *C, *pitch, *lds_bytes, or nonzero if the
+// computed FFT does not fit one workgroup.
+int generic_tiling(const pfk_pass *p, int *C, int *pitch, size_t *lds_bytes);
+
+}  // namespace pfk

@@ -77,6 +77,29 @@ typedef struct pfk_pass {
   int64_t src_elems, dst_elems;
 } pfk_pass;
 
+/* One task of a batched execution: one pass of one plan, with resolved
+ * device pointers.  `dep` is the index of the task that must have finished
+ * all its tiles before any tile of this task starts (-1: none). */
+typedef struct pfk_task {
+  pfk_pass pass;
+  const void *src;
+  void *dst;
+  int32_t C, pitch;       /* filled by pfk_batch_create */
+  int64_t ntiles;         /* filled by pfk_batch_create */
+  int64_t first_item;     /* filled by pfk_batch_create */
+  int32_t dep;
+  int32_t pad;
+} pfk_task;
+
+/* Batched execution of many passes: tasks grouped by pass level, one
+ * persistent launch per level walking a device work-list, all levels
+ * optionally captured into one hipGraph and replayed.  All tasks must
+ * share one precision. */
+typedef struct pfk_batch pfk_batch;
+pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_graph);
+int  pfk_batch_run(pfk_batch *b);
+void pfk_batch_destroy(pfk_batch *b);
+
 /* Device/runtime plumbing.  All return 0 on success, nonzero on error. */
 int  pfk_init(void);
 int  pfk_device_count(void);

@@ -41,6 +41,7 @@ typedef struct arrgeo {
 } arrgeo;
 
 struct pa_plan {
+  uint64_t id;              /* unique per created plan (batch cache key) */
   int prec;
   int rnk_n;
   INT n[PA_MAXRNK], ni[PA_MAXRNK], no[PA_MAXRNK];
@@ -268,6 +269,8 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
 
   pa_plan *p = (pa_plan *)calloc(1, sizeof(pa_plan));
   if (!p) return NULL;
+  static uint64_t next_id = 1;
+  p->id = next_id++;
   p->prec = prec;
   p->rnk_n = rnk_n;
   p->howmany = howmany;
@@ -458,8 +461,11 @@ fail:
   return NULL;
 }
 
+static void batch_cache_forget(uint64_t id);
+
 void pa_plan_destroy(pa_plan *p) {
   if (!p) return;
+  batch_cache_forget(p->id);
   free_plan_memory(p);
   free(p);
 }
@@ -537,9 +543,122 @@ static void run(const pa_plan *p, void *in, void *out, int adjoint) {
 void pa_plan_execute(const pa_plan *p, void *in, void *out) { run(p, in, out, 0); }
 void pa_plan_execute_adjoint(const pa_plan *p, void *in, void *out) { run(p, in, out, 1); }
 
+/* ---- batched execution ----------------------------------------------
+ * All passes of all plans become tasks of one persistent device launch
+ * (kernels_batch.hip): tasks are ordered by pass level (every plan's pass
+ * 0, then every plan's pass 1, ...), each depending on the same plan's
+ * previous pass.  The launch is captured once into a hipGraph per set of
+ * plans and replayed on later calls with the same plans. */
+#define PA_BATCH_CACHE 4
+typedef struct {
+  int used;
+  int count;
+  uint64_t *ids;
+  void **ins, **outs;
+  pfk_batch *b;
+  uint64_t stamp;
+} batch_entry;
+static batch_entry g_bcache[PA_BATCH_CACHE];
+static uint64_t g_bstamp = 0;
+
+static void batch_entry_clear(batch_entry *e) {
+  pfk_batch_destroy(e->b);
+  free(e->ids);
+  free(e->ins);
+  free(e->outs);
+  memset(e, 0, sizeof(*e));
+}
+
+static void batch_cache_forget(uint64_t id) {
+  for (int i = 0; i < PA_BATCH_CACHE; i++) {
+    batch_entry *e = &g_bcache[i];
+    if (!e->used) continue;
+    for (int k = 0; k < e->count; k++)
+      if (e->ids[k] == id) { batch_entry_clear(e); break; }
+  }
+}
+
+static pfk_batch *batch_build(const pa_plan *const *plans, int count) {
+  int maxp = 0, ntasks = 0;
+  for (int i = 0; i < count; i++) {
+    if (plans[i]->npass > maxp) maxp = plans[i]->npass;
+    ntasks += plans[i]->npass;
+  }
+  pfk_task *tasks = (pfk_task *)calloc((size_t)ntasks, sizeof(pfk_task));
+  int *last = (int *)malloc(sizeof(int) * (size_t)count);
+  for (int i = 0; i < count; i++) last[i] = -1;
+  int k = 0;
+  for (int lev = 0; lev < maxp; lev++)
+    for (int i = 0; i < count; i++) {
+      const pa_plan *p = plans[i];
+      if (lev >= p->npass) continue;
+      void *bufs[4] = {p->in, p->out, p->scratch[0], p->scratch[1]};
+      tasks[k].pass = p->pass[lev];
+      tasks[k].src = bufs[p->src_role[lev]];
+      tasks[k].dst = bufs[p->dst_role[lev]];
+      tasks[k].dep = last[i];
+      last[i] = k;
+      k++;
+    }
+  const char *ev = getenv("PFFT_AMD_BATCH_GRAPH");
+  const int use_graph = ev ? (ev[0] == '1') : 1;
+  pfk_batch *b = pfk_batch_create(tasks, ntasks, use_graph);
+  free(tasks);
+  free(last);
+  return b;
+}
+
 void pa_plans_execute_batch(const pa_plan *const *plans, int count) {
-  for (int i = 0; i < count; i++)
-    if (plans[i]) run(plans[i], plans[i]->in, plans[i]->out, 0);
+  if (count <= 0 || !plans) return;
+  int uniform = 1;
+  for (int i = 0; i < count; i++) {
+    if (!plans[i]) { fprintf(stderr, "!!! Error: Can not execute PFFT Plan == NULL !!!\n"); return; }
+    if (plans[i]->prec != plans[0]->prec || !pfk_is_device_ptr(plans[i]->in) ||
+        !pfk_is_device_ptr(plans[i]->out))
+      uniform = 0;
+  }
+  if (!uniform) {
+    /* host arrays or mixed precisions: one plan after another */
+    for (int i = 0; i < count; i++) run(plans[i], plans[i]->in, plans[i]->out, 0);
+    return;
+  }
+  batch_entry *hit = NULL;
+  for (int i = 0; i < PA_BATCH_CACHE && !hit; i++) {
+    batch_entry *e = &g_bcache[i];
+    if (!e->used || e->count != count) continue;
+    int same = 1;
+    for (int k = 0; k < count && same; k++)
+      same = e->ids[k] == plans[k]->id && e->ins[k] == plans[k]->in && e->outs[k] == plans[k]->out;
+    if (same) hit = e;
+  }
+  if (!hit) {
+    batch_entry *slot = &g_bcache[0];
+    for (int i = 0; i < PA_BATCH_CACHE; i++) {
+      if (!g_bcache[i].used) { slot = &g_bcache[i]; break; }
+      if (g_bcache[i].stamp < slot->stamp) slot = &g_bcache[i];
+    }
+    if (slot->used) batch_entry_clear(slot);
+    pfk_batch *b = batch_build(plans, count);
+    if (!b) {
+      pa_error("batch setup failed (%s); executing plans one by one", pfk_last_error());
+      for (int i = 0; i < count; i++) run(plans[i], plans[i]->in, plans[i]->out, 0);
+      return;
+    }
+    slot->used = 1;
+    slot->count = count;
+    slot->b = b;
+    slot->ids = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)count);
+    slot->ins = (void **)malloc(sizeof(void *) * (size_t)count);
+    slot->outs = (void **)malloc(sizeof(void *) * (size_t)count);
+    for (int k = 0; k < count; k++) {
+      slot->ids[k] = plans[k]->id;
+      slot->ins[k] = plans[k]->in;
+      slot->outs[k] = plans[k]->out;
+    }
+    hit = slot;
+  }
+  hit->stamp = ++g_bstamp;
+  if (pfk_batch_run(hit->b)) pa_error("batched execution failed: %s", pfk_last_error());
 }
 
 /* Times every pass of the plan over `iters` back-to-back executions on

@@ -1,0 +1,91 @@
+"""Batched execution (configuration 4 shape): many 3-d c2c complex64
+transforms of uneven sizes (sides from {24,32,48,60,64,80,96,128}) run as
+one persistent work-list launch, captured into a hipGraph and replayed."""
+import os
+
+import numpy as np
+import pytest
+
+import pfft_amd as P
+from gpu_util import spec_of, oracle_out, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SIDES = [24, 32, 48, 60, 64, 80, 96, 128]
+
+
+def make_batch(count, seed, prec="f32", max_side=128, flags=0):
+    rng = np.random.default_rng(seed)
+    rc, comm = P.create_procmesh([1, 1])
+    sides = [s for s in SIDES if s <= max_side]
+    items = []
+    for _ in range(count):
+        n = [int(rng.choice(sides)) for _ in range(3)]
+        N = int(np.prod(n))
+        rdt = np.float32 if prec == "f32" else np.float64
+        x = rng.uniform(-1, 1, 2 * N).astype(rdt)
+        esz = np.dtype(rdt).itemsize
+        din = P.DeviceBuffer(2 * N * esz)
+        dout = P.DeviceBuffer(2 * N * esz)
+        plan = P.Plan("c2c", n, din.ptr, dout.ptr, flags=flags, comm=comm, prec=prec)
+        assert plan.ok
+        din.upload(x)
+        items.append((n, x, din, dout, plan))
+    return items
+
+
+def outputs(items, prec="f32"):
+    rdt = np.float32 if prec == "f32" else np.float64
+    return [it[3].download(rdt, 2 * int(np.prod(it[0]))).astype(np.float64) for it in items]
+
+
+def test_batch_matches_oracle_and_sequential():
+    items = make_batch(12, 1, max_side=64)
+    P.execute_batch([it[4] for it in items])
+    got = outputs(items)
+    for (n, x, _, _, _), g in zip(items, got):
+        want = oracle_out(spec_of("c2c", n), x.astype(np.float64))
+        assert rel_l2(g, want) < 2e-6, n
+    for it in items:
+        it[4].execute()
+    seq = outputs(items)
+    for a, b in zip(got, seq):
+        assert rel_l2(a, b) < 2e-6
+
+
+def test_batch_graph_replays_with_eager_work_between():
+    items = make_batch(40, 2)
+    plans = [it[4] for it in items]
+    for it in items:
+        it[4].execute()
+    eager = outputs(items)
+    other = make_batch(3, 99, max_side=48)
+    for rep in range(3):
+        P.execute_batch(plans)            # first call captures, later calls replay
+        got = outputs(items)
+        for a, b in zip(got, eager):
+            assert rel_l2(a, b) < 1e-6, rep
+        # unrelated eager work and changed inputs between replays
+        for it in other:
+            it[4].execute()
+        for it in items[:5]:
+            it[3].upload(np.zeros(2 * int(np.prod(it[0])), dtype=np.float32))
+
+
+def test_batch_without_graph_and_fp64():
+    os.environ["PFFT_AMD_BATCH_GRAPH"] = "0"
+    try:
+        items = make_batch(10, 3, max_side=60)
+        P.execute_batch([it[4] for it in items])
+        got = outputs(items)
+        for (n, x, _, _, _), g in zip(items, got):
+            want = oracle_out(spec_of("c2c", n), x.astype(np.float64))
+            assert rel_l2(g, want) < 2e-6
+    finally:
+        os.environ.pop("PFFT_AMD_BATCH_GRAPH", None)
+    items = make_batch(8, 4, prec="f64", max_side=48, flags=P.TRANSPOSED_OUT)
+    P.execute_batch([it[4] for it in items])
+    got = outputs(items, "f64")
+    for (n, x, _, _, _), g in zip(items, got):
+        want = oracle_out(spec_of("c2c", n, flags=P.TRANSPOSED_OUT), x)
+        assert rel_l2(g, want) < 1e-12
