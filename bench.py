@@ -38,6 +38,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--workload", default="cfg2",
+                    choices=["cfg2", "cfg3", "cfg4", "cfg5"],
+                    help="cfg2 (default): 512^3 FP64 r2c+c2r; cfg3: precision study "
+                         "1024x1024x512 c2c; cfg4: 384 uneven c2c batch; cfg5: oversampled "
+                         "shifted r2c forward+adjoint")
     return ap.parse_args()
 
 
@@ -85,6 +90,10 @@ def main():
         print("bench.py: this build runs on one GPU only (no sharding, no collectives); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "cfg2":
+        import bench_configs
+        bench_configs.run(args)
+        return
     import pfft_amd as P
     L = P.lib()
     L.pfft_amd_time_passes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

@@ -1,0 +1,168 @@
+"""Measurements of the secondary configurations (bench.py --workload ...).
+
+cfg3  precision study: 3-d c2c 1024x1024x512, complex64 input U[-1,1)^2
+      plus a 1e-3-RMS 1/|k|^2 spectral component; forward+backward in FP64
+      (plain), FP32 (plain Stockham) and the MFMA forms when available;
+      relative L2 / max error of the forward result vs the FP64 path.
+cfg4  384 3-d c2c complex64 transforms, sides drawn from
+      {24,32,48,60,64,80,96,128}: batched (one persistent launch per pass
+      level, replayed hipGraph) vs a per-transform execute loop.
+cfg5  oversampled shifted r2c ni=128^3 -> n=256^3, loss sum w_k |X_k|^2:
+      forward + adjoint (gradient) time, FP64 and FP32.
+Each prints one JSON line.  Inputs are generated on the GPU with torch
+(test-data plumbing) and resident in HBM before timing.
+"""
+import ctypes
+import json
+import math
+import time
+
+import numpy as np
+
+import pfft_amd as P
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def _time(fn, reps, warm=1):
+    for _ in range(warm):
+        fn()
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    _sync()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def cfg3(args):
+    import torch
+    dev = torch.device("cuda:0")
+    n = [1024, 1024, 512]
+    N = int(np.prod(n))
+    g = torch.Generator(device=dev).manual_seed(args.seed)
+    x = (torch.rand(n, generator=g, device=dev) * 2 - 1) + \
+        1j * (torch.rand(n, generator=g, device=dev) * 2 - 1)
+    x = x.to(torch.complex64)
+    # 1/|k|^2 spectral component with random phase, RMS = 1e-3 * RMS(x)
+    k0 = torch.fft.fftfreq(n[0], d=1.0 / n[0], device=dev)
+    k1 = torch.fft.fftfreq(n[1], d=1.0 / n[1], device=dev)
+    k2 = torch.fft.fftfreq(n[2], d=1.0 / n[2], device=dev)
+    kk = k0[:, None, None] ** 2 + k1[None, :, None] ** 2 + k2[None, None, :] ** 2
+    amp = torch.where(kk > 0, 1.0 / kk.clamp(min=1.0), torch.zeros_like(kk))
+    ph = torch.rand(n, generator=g, device=dev) * (2 * math.pi)
+    spec = (amp * torch.exp(1j * ph)).to(torch.complex64)
+    del amp, ph, kk
+    comp = torch.fft.ifftn(spec)
+    del spec
+    comp *= (1e-3 * x.abs().pow(2).mean().sqrt() / comp.abs().pow(2).mean().sqrt())
+    x = (x + comp).contiguous()
+    del comp
+    rc, comm = P.create_procmesh([1, 1])
+    res = {"workload": "cfg3 c2c 1024x1024x512 fwd+bwd", "forms": []}
+    # FP64 plain: the reference result
+    x64 = x.to(torch.complex128)
+    y64 = torch.empty_like(x64)
+    z64 = torch.empty_like(x64)
+    fw = P.Plan("c2c", n, x64.data_ptr(), y64.data_ptr(), comm=comm, prec="f64")
+    bw = P.Plan("c2c", n, y64.data_ptr(), z64.data_ptr(), sign=+1, comm=comm, prec="f64")
+    ms64 = _time(lambda: (fw.execute(), bw.execute()), 3)
+    flops = 2 * 5.0 * N * math.log2(N)
+    res["forms"].append({"form": "fp64 plain", "ms_fwd_bwd": round(ms64, 3),
+                         "GFLOPs": round(flops / (ms64 * 1e-3) / 1e9, 1),
+                         "rel_l2_fwd": 0.0, "max_err_fwd": 0.0,
+                         "roundtrip_rel_l2": float((z64 / N - x64).norm() / x64.norm())})
+    fw.destroy(); bw.destroy(); del z64
+    ref = y64
+    refn = ref.norm().item()
+    for form, extra in (("fp32 plain", 0), ("fp32 mfma radix-16", P.AMD_MFMA_F32),
+                        ("split-bf16 mfma", P.AMD_MFMA_BF16X2)):
+        y = torch.empty_like(x)
+        z = torch.empty_like(x)
+        fw = P.Plan("c2c", n, x.data_ptr(), y.data_ptr(), comm=comm, prec="f32", flags=extra)
+        bw = P.Plan("c2c", n, y.data_ptr(), z.data_ptr(), sign=+1, comm=comm, prec="f32",
+                    flags=extra)
+        if not (fw.ok and bw.ok):
+            res["forms"].append({"form": form, "status": "plan not available"})
+            continue
+        ms = _time(lambda: (fw.execute(), bw.execute()), 3)
+        fw.execute()
+        d = (y.to(torch.complex128) - ref)
+        bw.execute()
+        res["forms"].append({
+            "form": form, "ms_fwd_bwd": round(ms, 3),
+            "GFLOPs": round(flops / (ms * 1e-3) / 1e9, 1),
+            "speedup_vs_fp64": round(ms64 / ms, 3),
+            "rel_l2_fwd": float(d.norm().item() / refn),
+            "max_err_fwd": float(d.abs().max().item() / ref.abs().max().item()),
+            "roundtrip_rel_l2": float(((z.to(torch.complex128) / N) - x64).norm() / x64.norm())})
+        fw.destroy(); bw.destroy()
+        del y, z
+    print(json.dumps(res))
+
+
+def cfg4(args):
+    import torch
+    dev = torch.device("cuda:0")
+    sides = [24, 32, 48, 60, 64, 80, 96, 128]
+    rng = np.random.default_rng(args.seed)
+    rc, comm = P.create_procmesh([1, 1])
+    plans, bufs = [], []
+    pts = 0
+    for _ in range(384):
+        n = [int(rng.choice(sides)) for _ in range(3)]
+        pts += int(np.prod(n))
+        a = torch.rand(n, device=dev, dtype=torch.complex64)
+        b = torch.empty_like(a)
+        p = P.Plan("c2c", n, a.data_ptr(), b.data_ptr(), comm=comm, prec="f32")
+        assert p.ok
+        plans.append(p)
+        bufs.append((a, b))
+    ms_batch = _time(lambda: P.execute_batch(plans), 10, warm=2)
+    ms_loop = _time(lambda: [p.execute() for p in plans], 3)
+    by = 16.0 * pts
+    print(json.dumps({"workload": "cfg4 384 uneven c2c complex64 (seed %d)" % args.seed,
+                      "points": pts, "compulsory_bytes": by,
+                      "ms_batched_graph": round(ms_batch, 3),
+                      "ms_per_transform_loop": round(ms_loop, 3),
+                      "speedup": round(ms_loop / ms_batch, 2),
+                      "GBps_batched": round(by / (ms_batch * 1e-3) / 1e9, 1)}))
+
+
+def cfg5(args):
+    import torch
+    from pfft_amd.autograd import apply_plan
+    dev = torch.device("cuda:0")
+    out = {"workload": "cfg5 ni=128^3 -> n=256^3 r2c SHIFTED_IN|SHIFTED_OUT, forward+adjoint"}
+    rc, comm = P.create_procmesh([1, 1])
+    ni, n = 128, 256
+    for prec in ("f64", "f32"):
+        rdt = torch.float64 if prec == "f64" else torch.float32
+        cdt = torch.complex128 if prec == "f64" else torch.complex64
+        x = torch.randn((ni, ni, ni), device=dev, dtype=rdt)
+        X = torch.empty((n, n, n // 2 + 1), device=dev, dtype=cdt)
+        plan = P.Plan("r2c", [n] * 3, x.data_ptr(), X.data_ptr(), ni=[ni] * 3,
+                      flags=P.SHIFTED_IN | P.SHIFTED_OUT, comm=comm, prec=prec)
+        k01 = torch.arange(-n // 2, n // 2, device=dev, dtype=torch.float64)
+        k2 = torch.arange(-n // 2, 1, device=dev, dtype=torch.float64)
+        w = torch.exp(-(k01[:, None, None] ** 2 + k01[None, :, None] ** 2 +
+                        k2[None, None, :] ** 2) / (2 * 40.0 ** 2)).to(rdt)
+        gX = torch.empty_like(X)
+        gx = torch.empty_like(x)
+
+        def fwd_bwd():
+            plan.execute_on(x.data_ptr(), X.data_ptr())
+            torch.mul(X, 2 * w, out=gX)
+            plan.execute_adjoint(gX.data_ptr(), gx.data_ptr())
+        ms = _time(fwd_bwd, 10, warm=2)
+        msf = _time(lambda: plan.execute_on(x.data_ptr(), X.data_ptr()), 10, warm=2)
+        out[prec] = {"ms_forward": round(msf, 3), "ms_forward_plus_adjoint": round(ms, 3)}
+        plan.destroy()
+    print(json.dumps(out))
+
+
+def run(args):
+    {"cfg3": cfg3, "cfg4": cfg4, "cfg5": cfg5}[args.workload](args)

@@ -215,6 +215,11 @@ int pfk_sync(void) {
 
 int pfk_launch_pass(const pfk_pass *p, const void *src, void *dst) {
   if (pfk_init()) return 1;
+  if (!g_generic_only && p->mfma) {
+    const int m = launch_mfma(p, src, dst);
+    if (m == 0) return 0;
+    if (m < 0) return 1;
+  }
   const int r = launch_fast(p, src, dst);   // 0: launched by a specialised kernel
   if (r == 0) return 0;
   if (r < 0) return 1;

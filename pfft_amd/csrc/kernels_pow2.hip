@@ -114,6 +114,42 @@ __device__ __forceinline__ void st_stream(C2<T> *p, C2<T> v) {
 #endif
 }
 
+// in-place DFT-16 of v[0..15] as 4x4: DFT-4 over stride-4 groups, twiddle
+// W16^(r*q), DFT-4 across groups
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft16(C2<T> *v) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) dft4<SIGN>(v[r], v[r + 4], v[r + 8], v[r + 12]);
+  // v[r + 4q] now holds sub-DFT output q of group r; multiply by W16^(r*q)
+  const T c1 = (T)0.92387953251128675612818318939678828682L;  // cos(pi/8)
+  const T s1 = (T)0.38268343236508977172845998403039886676L;  // sin(pi/8)
+  const T h = (T)0.70710678118654752440084436210484903928L;
+  const T S = (T)SIGN;
+  // W16^1 = c1 + S i s1, W16^2 = h + S i h, W16^3 = s1 + S i c1,
+  // W16^4 = S i, W16^6 = -h + S i h, W16^9 = -c1 - S i s1
+  const C2<T> w1{c1, S * s1}, w2{h, S * h}, w3{s1, S * c1};
+  const C2<T> w6{-h, S * h}, w9{-c1, -S * s1};
+  v[5] = cmul(v[5], w1);
+  v[9] = cmul(v[9], w2);
+  v[13] = cmul(v[13], w3);
+  v[6] = cmul(v[6], w2);
+  v[10] = mul_si<SIGN>(v[10]);
+  v[14] = cmul(v[14], w6);
+  v[7] = cmul(v[7], w3);
+  v[11] = cmul(v[11], w6);
+  v[15] = cmul(v[15], w9);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dft4<SIGN>(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  // output k = q + 4*s sits in v[4q + s]: transpose the 4x4 index
+  C2<T> o[16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) o[q + 4 * t] = v[4 * q + t];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = o[k];
+}
+
 __device__ __forceinline__ int msign(const pfk_mod &m, int64_t j) {
   if (!m.on) return 1;
   const int64_t t = j + m.start;
@@ -123,17 +159,17 @@ __device__ __forceinline__ int msign(const pfk_mod &m, int64_t j) {
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
-// One Stockham stage of radix R on the thread's 8 registers.
-// b = t + i*T for i < 8/R; inputs m = i + r*(8/R); twiddle W_NF^(r*k*NF/(L*R)).
-// Returns the outputs in `v` at register m' = i + q*(8/R) and, unless
-// LAST, the LDS position of each output in `pos`.
-template <typename T, int NF, int R, int L, int SIGN, bool LAST>
+// One Stockham stage of radix R on the thread's E registers (E = 8 or 16,
+// T = NF/E threads per line).  Butterflies b = t + i*T for i < E/R; inputs
+// m = i + r*(E/R); twiddle W_NF^(r*k*NF/(L*R)), k = b mod L.  Outputs go to
+// register m' = i + q*(E/R) and, unless LAST, their LDS positions to `pos`.
+template <typename T, int NF, int E, int R, int L, int SIGN, bool LAST>
 __device__ __forceinline__ void stage(C2<T> *v, int *pos, const int t,
                                       const C2<T> *__restrict__ tw) {
-  constexpr int TT = NF / 8;
-  constexpr int NB = 8 / R;
+  constexpr int TT = NF / E;
+  constexpr int NB = E / R;
   constexpr int M = NF / (L * R);
-  C2<T> o[8];
+  C2<T> o[E];
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     const int b = t + i * TT;
@@ -145,7 +181,8 @@ __device__ __forceinline__ void stage(C2<T> *v, int *pos, const int t,
 #pragma unroll
       for (int r = 1; r < R; ++r) a[r] = cmul(a[r], tw[r * k * M]);
     }
-    if (R == 8) dft8<SIGN>(a);
+    if (R == 16) dft16<SIGN>(a);
+    else if (R == 8) dft8<SIGN>(a);
     else if (R == 4) dft4<SIGN>(a[0], a[1], a[2], a[3]);
     else dft2<SIGN>(a[0], a[1]);
 #pragma unroll
@@ -155,51 +192,46 @@ __device__ __forceinline__ void stage(C2<T> *v, int *pos, const int t,
     }
   }
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = o[m];
+  for (int m = 0; m < E; ++m) v[m] = o[m];
 }
 
 // Exchange through LDS: write v[m] at position pos[m], read back t + T*m.
-template <typename T, int NF, int C>
+template <typename T, int NF, int C, int E>
 __device__ __forceinline__ void exchange(C2<T> *v, const int *pos, C2<T> *lds, const int t,
                                          const int c) {
-  constexpr int TT = NF / 8;
+  constexpr int TT = NF / E;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) lds[pos[m] * C + c] = v[m];
+  for (int m = 0; m < E; ++m) lds[pos[m] * C + c] = v[m];
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = lds[(t + TT * m) * C + c];
+  for (int m = 0; m < E; ++m) v[m] = lds[(t + TT * m) * C + c];
   __syncthreads();
 }
 
 // Full length-NF FFT on registers; first stage input = v (positions t+T*m),
-// output X[t + T*m] in v[m].
-template <typename T, int NF, int C, int SIGN>
+// output X[t + T*m] in v[m].  Stages of radix E, then one of radix
+// NF / E^A if NF is not a power of E.
+template <typename T, int NF, int C, int E, int SIGN, int L>
+__device__ __forceinline__ void fft_stages(C2<T> *v, int *pos, C2<T> *lds, const int t,
+                                           const int c, const C2<T> *__restrict__ tw) {
+  constexpr int REM = NF / L;  // remaining length to transform
+  if constexpr (REM >= E) {
+    constexpr bool LAST = (REM == E);
+    stage<T, NF, E, E, L, SIGN, LAST>(v, pos, t, tw);
+    if constexpr (!LAST) {
+      exchange<T, NF, C, E>(v, pos, lds, t, c);
+      fft_stages<T, NF, C, E, SIGN, L * E>(v, pos, lds, t, c, tw);
+    }
+  } else if constexpr (REM > 1) {
+    stage<T, NF, E, REM, L, SIGN, true>(v, pos, t, tw);
+  }
+}
+
+template <typename T, int NF, int C, int E, int SIGN>
 __device__ __forceinline__ void fft_regs(C2<T> *v, C2<T> *lds, const int t, const int c,
                                          const C2<T> *__restrict__ tw) {
-  constexpr int LG = ilog2c(NF);
-  constexpr int A = LG / 3;   // radix-8 stages
-  constexpr int B = LG % 3;   // final radix-2^B stage
-  int pos[8];
-  // unrolled stage list (NF <= 4096: A <= 4)
-  if constexpr (A >= 1) {
-    stage<T, NF, 8, 1, SIGN, (A == 1 && B == 0)>(v, pos, t, tw);
-    if constexpr (!(A == 1 && B == 0)) exchange<T, NF, C>(v, pos, lds, t, c);
-  }
-  if constexpr (A >= 2) {
-    stage<T, NF, 8, 8, SIGN, (A == 2 && B == 0)>(v, pos, t, tw);
-    if constexpr (!(A == 2 && B == 0)) exchange<T, NF, C>(v, pos, lds, t, c);
-  }
-  if constexpr (A >= 3) {
-    stage<T, NF, 8, 64, SIGN, (A == 3 && B == 0)>(v, pos, t, tw);
-    if constexpr (!(A == 3 && B == 0)) exchange<T, NF, C>(v, pos, lds, t, c);
-  }
-  if constexpr (A >= 4) {
-    stage<T, NF, 8, 512, SIGN, (A == 4 && B == 0)>(v, pos, t, tw);
-    if constexpr (!(A == 4 && B == 0)) exchange<T, NF, C>(v, pos, lds, t, c);
-  }
-  constexpr int LA = (A == 0) ? 1 : (A == 1 ? 8 : (A == 2 ? 64 : (A == 3 ? 512 : 4096)));
-  if constexpr (B == 1) stage<T, NF, 2, LA, SIGN, true>(v, pos, t, tw);
-  if constexpr (B == 2) stage<T, NF, 4, LA, SIGN, true>(v, pos, t, tw);
+  int pos[E];
+  fft_stages<T, NF, C, E, SIGN, 1>(v, pos, lds, t, c, tw);
 }
 
 struct TileCtx {
@@ -239,11 +271,11 @@ __device__ __forceinline__ TileCtx tile_of(const pfk_pass &P, int C, int64_t til
 }
 
 // ------------------------------------------------------------------ c2c
-template <typename T, int NF, int C, int SIGN>
-__global__ void __launch_bounds__(C *NF / 8)
+template <typename T, int NF, int C, int E, int SIGN>
+__global__ void __launch_bounds__(C *NF / E)
 k_c2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
            T *__restrict__ dst) {
-  constexpr int TT = NF / 8;
+  constexpr int TT = NF / E;
   __shared__ C2<T> lds[NF * C];
   const int tid = threadIdx.x;
   const int c = tid % C, t = tid / C;
@@ -253,9 +285,9 @@ k_c2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
   C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
   const C2<T> *tw = reinterpret_cast<const C2<T> *>(P.tw);
   const int zin = (int)P.zin, nin = (int)P.nin, zout = (int)P.zout, nout = (int)P.nout;
-  C2<T> v[8];
+  C2<T> v[E];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
+  for (int m = 0; m < E; ++m) {
     const int j = t + TT * m - zin;
     C2<T> x{0, 0};
     if (valid && j >= 0 && j < nin) {
@@ -266,10 +298,10 @@ k_c2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
     }
     v[m] = x;
   }
-  fft_regs<T, NF, C, SIGN>(v, lds, t, c, tw);
+  fft_regs<T, NF, C, E, SIGN>(v, lds, t, c, tw);
   if (valid) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < E; ++m) {
       const int k = t + TT * m - zout;
       if (k >= 0 && k < nout) {
         const T f = (T)msign(P.omod, k);
@@ -282,11 +314,11 @@ k_c2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
 // ------------------------------------------------------------------ r2c
 // real line of n = 2*NF contiguous samples (sin == 1, no embed) ->
 // kept half-spectrum outputs [zout, zout+nout) of the n/2+1.
-template <typename T, int NF, int C, int SIGN>
-__global__ void __launch_bounds__(C *NF / 8)
+template <typename T, int NF, int C, int E, int SIGN>
+__global__ void __launch_bounds__(C *NF / E)
 k_r2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
            T *__restrict__ dst) {
-  constexpr int TT = NF / 8;
+  constexpr int TT = NF / E;
   constexpr int H = NF;  // h = n/2
   __shared__ C2<T> lds[NF * C];
   const int tid = threadIdx.x;
@@ -297,9 +329,9 @@ k_r2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
   C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
   const C2<T> *tw = reinterpret_cast<const C2<T> *>(P.tw);
   const C2<T> *rtw = reinterpret_cast<const C2<T> *>(P.rtw);
-  C2<T> v[8];
+  C2<T> v[E];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
+  for (int m = 0; m < E; ++m) {
     const int p = t + TT * m;
     C2<T> x{0, 0};
     if (valid) {
@@ -311,33 +343,33 @@ k_r2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
     }
     v[m] = x;
   }
-  fft_regs<T, NF, C, SIGN>(v, lds, t, c, tw);
+  fft_regs<T, NF, C, E, SIGN>(v, lds, t, c, tw);
   // mirror exchange: Z_{h-k}
 #pragma unroll
-  for (int m = 0; m < 8; ++m) lds[(t + TT * m) * C + c] = v[m];
+  for (int m = 0; m < E; ++m) lds[(t + TT * m) * C + c] = v[m];
   __syncthreads();
   const int zout = (int)P.zout, nout = (int)P.nout;
   if (valid) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < E; ++m) {
       const int k = t + TT * m;
       const C2<T> a = v[m];
       const C2<T> b = lds[((H - k) & (H - 1)) * C + c];
-      // E = (a + conj b)/2, O = -i (a - conj b)/2, X = E + W^k O
-      const C2<T> E{(a.x + b.x) * (T)0.5, (a.y - b.y) * (T)0.5};
+      // Ev = (a + conj b)/2, O = -i (a - conj b)/2, X = Ev + W^k O
+      const C2<T> Ev{(a.x + b.x) * (T)0.5, (a.y - b.y) * (T)0.5};
       const C2<T> O{(a.y + b.y) * (T)0.5, -(a.x - b.x) * (T)0.5};
       const int ko = k - zout;
       if (ko >= 0 && ko < nout) {
         T f = (T)msign(P.omod, ko);
         if (P.hw == 2 && k != 0) f *= (T)2;
         const C2<T> WO = cmul(O, rtw[k]);
-        st_stream(d + (int64_t)ko * P.sout, C2<T>{(E.x + WO.x) * f, (E.y + WO.y) * f});
+        st_stream(d + (int64_t)ko * P.sout, C2<T>{(Ev.x + WO.x) * f, (Ev.y + WO.y) * f});
       }
       if (k == 0) {  // Nyquist X_h = E_0 - O_0 (W^h = -1)
         const int kn = H - zout;
         if (kn >= 0 && kn < nout) {
           const T f = (T)msign(P.omod, kn);
-          d[(int64_t)kn * P.sout] = C2<T>{(E.x - O.x) * f, (E.y - O.y) * f};
+          d[(int64_t)kn * P.sout] = C2<T>{(Ev.x - O.x) * f, (Ev.y - O.y) * f};
         }
       }
     }
@@ -347,11 +379,11 @@ k_r2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
 // ------------------------------------------------------------------ c2r
 // half spectrum (embedded at zin) -> real line of n = 2*NF contiguous
 // samples (sout == 1, no trunc).
-template <typename T, int NF, int C, int SIGN>
-__global__ void __launch_bounds__(C *NF / 8)
+template <typename T, int NF, int C, int E, int SIGN>
+__global__ void __launch_bounds__(C *NF / E)
 k_c2r_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
            T *__restrict__ dst) {
-  constexpr int TT = NF / 8;
+  constexpr int TT = NF / E;
   constexpr int H = NF;
   __shared__ C2<T> lds[(NF + 1) * C];
   const int tid = threadIdx.x;
@@ -364,9 +396,9 @@ k_c2r_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
   const C2<T> *rtw = reinterpret_cast<const C2<T> *>(P.rtw);
   const int zin = (int)P.zin, nin = (int)P.nin;
   // load X_k, k = t + T*m, and X_h (thread t == 0) into LDS
-  C2<T> v[8];
+  C2<T> v[E];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
+  for (int m = 0; m < E; ++m) {
     const int k = t + TT * m;
     const int j = k - zin;
     C2<T> x{0, 0};
@@ -392,7 +424,7 @@ k_c2r_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
   __syncthreads();
   // Z_k = E_k + i O_k
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
+  for (int m = 0; m < E; ++m) {
     const int k = t + TT * m;
     const C2<T> a = v[m];
     C2<T> z;
@@ -401,17 +433,17 @@ k_c2r_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
       z = C2<T>{a.x + xh, a.x - xh};
     } else {
       const C2<T> b = lds[(H - k) * C + c];
-      const C2<T> E{a.x + b.x, a.y - b.y};
+      const C2<T> Ev{a.x + b.x, a.y - b.y};
       const C2<T> O = cmul(C2<T>{a.x - b.x, a.y + b.y}, rtw[k]);
-      z = C2<T>{E.x - O.y, E.y + O.x};
+      z = C2<T>{Ev.x - O.y, Ev.y + O.x};
     }
     v[m] = z;
   }
   __syncthreads();
-  fft_regs<T, NF, C, SIGN>(v, lds, t, c, tw);
+  fft_regs<T, NF, C, E, SIGN>(v, lds, t, c, tw);
   if (valid) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < E; ++m) {
       const int p = t + TT * m;
       C2<T> x = v[m];
       if (P.omod.on) {
@@ -424,7 +456,7 @@ k_c2r_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
 }
 
 // ------------------------------------------------------------------ dispatch
-template <typename T, int NF, int C>
+template <typename T, int NF, int C, int E>
 static int launch_nf(const pfk_pass *p, const void *src, void *dst) {
   int64_t ntiles = (p->text + C - 1) / C;
   for (int i = 0; i < p->nb; ++i) ntiles *= p->bext[i];
@@ -434,22 +466,22 @@ static int launch_nf(const pfk_pass *p, const void *src, void *dst) {
   const int64_t chunk = (int64_t)1 << 30;
   for (int64_t base = 0; base < ntiles; base += chunk) {
     const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
-    dim3 grid((unsigned)nblk), block(C * NF / 8);
+    dim3 grid((unsigned)nblk), block(C * NF / E);
     if (p->kind == PFK_C2C) {
       if (p->sign < 0)
-        hipLaunchKernelGGL((k_c2c_pow2<T, NF, C, -1>), grid, block, 0, g_stream, *p, base, s, d);
+        hipLaunchKernelGGL((k_c2c_pow2<T, NF, C, E, -1>), grid, block, 0, g_stream, *p, base, s, d);
       else
-        hipLaunchKernelGGL((k_c2c_pow2<T, NF, C, +1>), grid, block, 0, g_stream, *p, base, s, d);
+        hipLaunchKernelGGL((k_c2c_pow2<T, NF, C, E, +1>), grid, block, 0, g_stream, *p, base, s, d);
     } else if (p->kind == PFK_R2C) {
       if (p->sign < 0)
-        hipLaunchKernelGGL((k_r2c_pow2<T, NF, C, -1>), grid, block, 0, g_stream, *p, base, s, d);
+        hipLaunchKernelGGL((k_r2c_pow2<T, NF, C, E, -1>), grid, block, 0, g_stream, *p, base, s, d);
       else
-        hipLaunchKernelGGL((k_r2c_pow2<T, NF, C, +1>), grid, block, 0, g_stream, *p, base, s, d);
+        hipLaunchKernelGGL((k_r2c_pow2<T, NF, C, E, +1>), grid, block, 0, g_stream, *p, base, s, d);
     } else {
       if (p->sign < 0)
-        hipLaunchKernelGGL((k_c2r_pow2<T, NF, C, -1>), grid, block, 0, g_stream, *p, base, s, d);
+        hipLaunchKernelGGL((k_c2r_pow2<T, NF, C, E, -1>), grid, block, 0, g_stream, *p, base, s, d);
       else
-        hipLaunchKernelGGL((k_c2r_pow2<T, NF, C, +1>), grid, block, 0, g_stream, *p, base, s, d);
+        hipLaunchKernelGGL((k_c2r_pow2<T, NF, C, E, +1>), grid, block, 0, g_stream, *p, base, s, d);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { set_error("pfk_launch_pass(pow2)", e); return -1; }
@@ -458,19 +490,42 @@ static int launch_nf(const pfk_pass *p, const void *src, void *dst) {
 }
 
 static int g_cols512 = 8;
+static int g_alt1024 = 0;
 
-template <typename T>
-static int launch_pow2(const pfk_pass *p, const void *src, void *dst) {
+// Tile width C (consecutive lines per workgroup) gives 128-byte segments:
+// C = 8 for 16-byte FP64 complex, C = 16 for 8-byte FP32 complex (C = 4 / 8
+// at the longest lengths, where the tile would exceed 128 KiB of LDS).
+// E = 16 elements per thread from NF = 1024 on keeps the workgroup at
+// <= 1024 threads.
+static int launch_pow2_f64(const pfk_pass *p, const void *src, void *dst) {
   switch (p->nf) {
-    case 64: return launch_nf<T, 64, 8>(p, src, dst);
-    case 128: return launch_nf<T, 128, 8>(p, src, dst);
-    case 256: return launch_nf<T, 256, 8>(p, src, dst);
+    case 64: return launch_nf<double, 64, 8, 8>(p, src, dst);
+    case 128: return launch_nf<double, 128, 8, 8>(p, src, dst);
+    case 256: return launch_nf<double, 256, 8, 8>(p, src, dst);
     case 512:
-      if (p->kind == PFK_C2C && g_cols512 == 4) return launch_nf<T, 512, 4>(p, src, dst);
-      if (p->kind == PFK_C2C && g_cols512 == 16) return launch_nf<T, 512, 16>(p, src, dst);
-      return launch_nf<T, 512, 8>(p, src, dst);
-    case 1024: return launch_nf<T, 1024, 4>(p, src, dst);
-    case 2048: return launch_nf<T, 2048, 2>(p, src, dst);
+      if (p->kind == PFK_C2C && g_cols512 == 4) return launch_nf<double, 512, 4, 8>(p, src, dst);
+      if (p->kind == PFK_C2C && g_cols512 == 16) return launch_nf<double, 512, 16, 8>(p, src, dst);
+      return launch_nf<double, 512, 8, 8>(p, src, dst);
+    case 1024:
+      if (g_alt1024) return launch_nf<double, 1024, 4, 16>(p, src, dst);
+      return launch_nf<double, 1024, 8, 16>(p, src, dst);
+    case 2048: return launch_nf<double, 2048, 4, 16>(p, src, dst);
+    case 4096: return launch_nf<double, 4096, 2, 16>(p, src, dst);
+    default: return 1;
+  }
+}
+
+static int launch_pow2_f32(const pfk_pass *p, const void *src, void *dst) {
+  switch (p->nf) {
+    case 64: return launch_nf<float, 64, 16, 8>(p, src, dst);
+    case 128: return launch_nf<float, 128, 16, 8>(p, src, dst);
+    case 256: return launch_nf<float, 256, 16, 8>(p, src, dst);
+    case 512: return launch_nf<float, 512, 16, 8>(p, src, dst);
+    case 1024:
+      if (g_alt1024) return launch_nf<float, 1024, 8, 16>(p, src, dst);
+      return launch_nf<float, 1024, 16, 16>(p, src, dst);
+    case 2048: return launch_nf<float, 2048, 8, 16>(p, src, dst);
+    case 4096: return launch_nf<float, 4096, 4, 16>(p, src, dst);
     default: return 1;
   }
 }
@@ -482,12 +537,14 @@ int launch_fast(const pfk_pass *p, const void *src, void *dst) {
     if (!init) {
       const char *e = getenv("PFFT_AMD_COLS512");
       if (e) g_cols512 = atoi(e);
+      e = getenv("PFFT_AMD_ALT1024");
+      if (e) g_alt1024 = atoi(e);
       init = 1;
     }
   }
   if (p->kind == PFK_COPY) return 1;
   const int64_t nf = p->nf;
-  if (nf < 64 || nf > 2048 || (nf & (nf - 1))) return 1;
+  if (nf < 64 || nf > 4096 || (nf & (nf - 1))) return 1;
   if (p->kind == PFK_R2C) {
     // real input: contiguous, not embedded
     if (p->n != 2 * nf || p->sin != 1 || p->zin != 0 || p->nin != p->n) return 1;
@@ -498,8 +555,8 @@ int launch_fast(const pfk_pass *p, const void *src, void *dst) {
     if (p->hw == 2) return 1;
   }
   if (p->kind == PFK_C2C && p->hw) return 1;
-  const int r = (p->prec == PFK_F64) ? launch_pow2<double>(p, src, dst)
-                                     : launch_pow2<float>(p, src, dst);
+  const int r = (p->prec == PFK_F64) ? launch_pow2_f64(p, src, dst)
+                                     : launch_pow2_f32(p, src, dst);
   return r;   // 0 launched, 1 not covered, -1 launch error
 }
 

@@ -67,7 +67,7 @@ typedef struct pfk_pass {
   int64_t bext[PFK_MAXB], bsin[PFK_MAXB], bsout[PFK_MAXB];
   int64_t text, tsin, tsout;   /* tile dimension (text==1: none) */
   int32_t xcd_remap;  /* map consecutive tiles to one XCD (see kernels_pow2.hip) */
-  int32_t pad1;
+  int32_t mfma;       /* complex64 c2c arithmetic form: 0 plain, 1 FP32 MFMA, 2 split-BF16 MFMA */
   /* device twiddle tables (owned by the host plan):
    *   tw  : W_nf^m = exp(sign*2*pi*i*m/nf), m in [0,nf)
    *   rtw : W_n^k  = exp(sign*2*pi*i*k/n),  k in [0,n/2] (even r2c/c2r only) */

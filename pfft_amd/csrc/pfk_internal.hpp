@@ -25,5 +25,6 @@ void set_error_msg(const char *msg);
 // pass shape is not covered (the caller then uses the general kernel).
 int launch_fast(const pfk_pass *p, const void *src, void *dst);
 int launch_generic(const pfk_pass *p, const void *src, void *dst);
+int launch_mfma(const pfk_pass *p, const void *src, void *dst);
 
 }  // namespace pfk

@@ -253,6 +253,14 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     return NULL;
   }
   if (tf & TF_R2R) { pa_error("r2r transforms are not supported"); return NULL; }
+  if ((pfft_flags & PFFT_AMD_MFMA_F32) && (pfft_flags & PFFT_AMD_MFMA_BF16X2)) {
+    pa_error("choose one of PFFT_AMD_MFMA_F32 and PFFT_AMD_MFMA_BF16X2");
+    return NULL;
+  }
+  if ((pfft_flags & (PFFT_AMD_MFMA_F32 | PFFT_AMD_MFMA_BF16X2)) && prec != PFK_F32) {
+    pa_error("the MFMA arithmetic forms are for complex64 (pfftf_) plans");
+    return NULL;
+  }
   if (mesh.size != 1) { pa_error("process mesh must have one process"); return NULL; }
   if (howmany < 1) return NULL;
   for (int t = 0; t < rnk_n; t++) {
@@ -382,6 +390,7 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     else if (skipped) kind = PFK_COPY;
     ps->kind = kind;
     ps->xcd_remap = xcd_remap;
+    ps->mfma = (pfft_flags & PFFT_AMD_MFMA_F32) ? 1 : (pfft_flags & PFFT_AMD_MFMA_BF16X2) ? 2 : 0;
     ps->prec = prec;
     ps->sign = sign;
     ps->n = n[d];

@@ -206,3 +206,42 @@ def test_pow2_howmany_fp32(kind, flags):
     sign = +1 if kind == "c2r" else -1
     check(kind, [64, 128, 64], howmany=2, sign=sign, flags=flags)
     check(kind, [64, 128, 64], sign=sign, flags=flags, prec="f32")
+
+
+# ---- long power-of-two lines: 16 elements per thread, radix-16 stages ----
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("n,flags", [([1024, 8, 16], 0), ([16, 1024, 8], TO), ([8, 16, 2048], TI),
+                                     ([4096, 4, 8], TO), ([4, 8, 4096], 0)])
+def test_long_pow2_c2c(n, flags, prec):
+    check("c2c", n, flags=flags, prec=prec)
+    check("c2c", n, flags=flags, prec=prec, sign=+1)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("n", [[8, 4, 2048], [4, 4, 8192], [16, 2, 4096]])
+def test_long_pow2_real(n, prec):
+    check("r2c", n, flags=TO, prec=prec)
+    check("c2r", n, flags=TI, sign=+1, prec=prec)
+
+
+# ---- complex64 MFMA arithmetic forms (radix-16 stages on the matrix cores) ----
+@pytest.mark.parametrize("form,tol", [(P.AMD_MFMA_F32, 5e-6), (P.AMD_MFMA_BF16X2, 1e-4)])
+@pytest.mark.parametrize("n,flags", [([512, 64, 256], 0), ([256, 512, 32], TO),
+                                     ([32, 256, 1024], TI), ([16, 2048, 8], 0),
+                                     ([4096, 4, 8], TO), ([40, 1024, 24], 0)])
+def test_mfma_forms(n, flags, form, tol):
+    for sign in (-1, +1):
+        spec = spec_of("c2c", n, flags=flags, sign=sign)
+        x = random_input(spec, 5).astype(np.float32).astype(np.float64)
+        got = run_gpu(spec, x, prec="f32", plan_flags_extra=form)
+        want = oracle_out(spec, x)
+        err = rel_l2(got, want)
+        assert err <= tol, (n, flags, form, sign, err)
+
+
+def test_mfma_form_rejected_for_fp64():
+    import pfft_amd as PP
+    rc, comm = PP.create_procmesh([1, 1])
+    d = PP.DeviceBuffer(16 * 64)
+    p = PP.Plan("c2c", [4, 4, 4], d.ptr, d.ptr, flags=PP.AMD_MFMA_F32, comm=comm, prec="f64")
+    assert not p.ok
