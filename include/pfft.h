@@ -285,6 +285,21 @@ int pfft_amd_time_passes(const void *plan, void *in, void *out, int iters, doubl
                          double *bytes);
 int pfft_amd_describe_plan(const void *plan, char *buf, size_t len);
 double pfft_amd_copy_bandwidth(size_t bytes, int iters);
+/* Wisdom: the launch variants the planner chose by timing (flags
+ * PFFT_MEASURE, PFFT_PATIENT, PFFT_EXHAUSTIVE; none with PFFT_ESTIMATE),
+ * keyed by pass geometry.  Export writes the process-wide cache to a text
+ * file, import merges a file into it (0 on success), forget empties it;
+ * count is the number of cached selections and tune_timings the number of
+ * candidate timings performed so far.  PFFT_AMD_WISDOM=<file> imports the
+ * file before the first tuned plan and rewrites it as selections are
+ * added.  FFTW offers the same services as fftw_export_wisdom_to_filename
+ * and friends; the reference exposes none (kernel/ipfft.h:44-50 only
+ * forgets FFTW's wisdom). */
+int pfft_amd_export_wisdom(const char *filename);
+int pfft_amd_import_wisdom(const char *filename);
+void pfft_amd_forget_wisdom(void);
+int pfft_amd_wisdom_count(void);
+int pfft_amd_tune_timings(void);
 /* Library version string. */
 const char *pfft_amd_version(void);
 /* Last error message of the device runtime (empty string if none). */

@@ -103,6 +103,9 @@ def _declare(L):
         getattr(L, P + "destroy_plan").restype = None
         getattr(L, P + "malloc").argtypes = [ctypes.c_size_t]
         getattr(L, P + "malloc").restype = VP
+        for nm in ("alloc_real", "alloc_complex"):
+            getattr(L, P + nm).argtypes = [INT]
+            getattr(L, P + nm).restype = VP
         getattr(L, P + "free").argtypes = [VP]
         getattr(L, P + "free").restype = None
         getattr(L, P + "init").restype = None
@@ -133,6 +136,14 @@ def _declare(L):
     L.pfft_amd_memcpy_to_host.restype = ctypes.c_int
     L.pfft_amd_version.restype = ctypes.c_char_p
     L.pfft_amd_last_error.restype = ctypes.c_char_p
+    for nm in ("export_wisdom", "import_wisdom"):
+        getattr(L, "pfft_amd_" + nm).argtypes = [ctypes.c_char_p]
+        getattr(L, "pfft_amd_" + nm).restype = ctypes.c_int
+    L.pfft_amd_forget_wisdom.restype = None
+    L.pfft_amd_wisdom_count.restype = ctypes.c_int
+    L.pfft_amd_tune_timings.restype = ctypes.c_int
+    L.pfft_amd_describe_plan.argtypes = [VP, ctypes.c_char_p, ctypes.c_size_t]
+    L.pfft_amd_describe_plan.restype = ctypes.c_int
 
 
 def ivec(v):
@@ -225,6 +236,30 @@ def last_error():
     return lib().pfft_amd_last_error().decode()
 
 
+# ---------------------------------------------------------------- wisdom
+def export_wisdom(path):
+    """Writes the planner's launch-variant selections to `path` (0 = ok)."""
+    return lib().pfft_amd_export_wisdom(os.fsencode(path))
+
+
+def import_wisdom(path):
+    """Merges a wisdom file into the process-wide cache (0 = ok)."""
+    return lib().pfft_amd_import_wisdom(os.fsencode(path))
+
+
+def forget_wisdom():
+    lib().pfft_amd_forget_wisdom()
+
+
+def wisdom_count():
+    return lib().pfft_amd_wisdom_count()
+
+
+def tune_timings():
+    """Number of candidate timings the planner has performed."""
+    return lib().pfft_amd_tune_timings()
+
+
 # ---------------------------------------------------------------- plans
 class Plan:
     """A pfft plan handle (pfft_plan_many_dft* / pfftf_*)."""
@@ -255,6 +290,12 @@ class Plan:
 
     def execute(self):
         getattr(lib(), self.P + "execute")(self.handle)
+
+    def describe(self):
+        """One line per device pass: geometry and the kernel that runs it."""
+        buf = ctypes.create_string_buffer(8192)
+        lib().pfft_amd_describe_plan(self.handle, buf, 8192)
+        return buf.value.decode().strip().split("\n")
 
     def execute_on(self, in_ptr, out_ptr):
         suffix = {"c2c": "dft", "r2c": "dft_r2c", "c2r": "dft_c2r"}[self.kind]

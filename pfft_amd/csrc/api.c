@@ -178,3 +178,9 @@ int pfft_amd_describe_plan(const void *plan, char *buf, size_t len) {
   return pa_plan_describe((const pa_plan *)plan, buf, len);
 }
 double pfft_amd_copy_bandwidth(size_t bytes, int iters) { return pfk_copy_bandwidth(bytes, iters); }
+
+int pfft_amd_export_wisdom(const char *filename) { return pa_wisdom_export(filename); }
+int pfft_amd_import_wisdom(const char *filename) { return pa_wisdom_import(filename); }
+void pfft_amd_forget_wisdom(void) { pa_wisdom_forget(); }
+int pfft_amd_wisdom_count(void) { return pa_wisdom_count(); }
+int pfft_amd_tune_timings(void) { return pa_tune_timings(); }

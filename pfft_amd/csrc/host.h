@@ -67,6 +67,16 @@ int pa_plan_time_passes(const pa_plan *p, void *in, void *out, int iters, double
 int pa_plan_npasses(const pa_plan *p);
 int pa_plan_describe(const pa_plan *p, char *buf, size_t len);
 
+/* ---- autotuner and wisdom (tune.c) ---- */
+int pa_tune_lookup(pfk_pass *ps, unsigned effort);
+int pa_tune_measure(pfk_pass *ps, const void *src, void *dst, unsigned effort);
+double pa_pass_bytes(const pfk_pass *ps);
+int pa_wisdom_export(const char *filename);
+int pa_wisdom_import(const char *filename);
+void pa_wisdom_forget(void);
+int pa_wisdom_count(void);
+int pa_tune_timings(void);
+
 /* ---- twiddles (twiddle.c) ---- */
 /* exp(sign*2*pi*i*m/N) for m in [0, count), rounded to double, computed in
  * long double with exact integer quadrant reduction. */

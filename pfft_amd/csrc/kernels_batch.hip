@@ -1,33 +1,44 @@
 // kernels_batch.hip -- persistent work-list execution of many passes.
 //
 // A batch is a list of tasks (one pass of one plan each) grouped by pass
-// level: every plan's pass 0, then every plan's pass 1, and so on.  Each
-// level runs as one persistent launch over a device work-list: the tiles
-// of all tasks of the level form one item sequence, and a fixed grid of
-// resident workgroups walks it with a static stride (workgroup b takes
-// items b, b + grid, b + 2*grid, ...).  There are no waits and no atomics
-// inside a launch; the order between levels (pass p+1 of a plan reads what
-// pass p wrote) is given by the launch boundaries.  All level launches are
-// captured into one hipGraph and replayed.
+// level: every plan's pass 0, then every plan's pass 1, and so on.  Within
+// a level the tasks are grouped by the kernel that runs them (the general
+// tile kernel, or the short-length kernel of one length), and each group
+// runs as one persistent launch over a device work-list: the tiles of the
+// group's tasks form one item sequence, and a fixed grid of resident
+// workgroups walks it with a static stride (workgroup b takes items b,
+// b + grid, b + 2*grid, ...).  There are no waits and no atomics inside a
+// launch; the order between levels (pass p+1 of a plan reads what pass p
+// wrote) is given by the launch boundaries.  All launches are captured into
+// one hipGraph and replayed.  Separate kernels per group keep each
+// kernel's register allocation (and so its occupancy) its own.
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include "pass_tile.hpp"
+#include "small_tile.hpp"
 
 namespace pfk {
 
+// task of work-list item g within tasks [t0, t1)
+__device__ __forceinline__ int find_task(const pfk_task *__restrict__ tasks, int t0, int t1,
+                                         int64_t g) {
+  int lo = t0, hi = t1 - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].first_item <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_batch_level(const pfk_task *__restrict__ tasks, const int t0, const int t1,
-              const int64_t item0, const int64_t item1) {
+k_batch_generic(const pfk_task *__restrict__ tasks, const int t0, const int t1,
+                const int64_t item0, const int64_t item1) {
   extern __shared__ unsigned char smem_raw[];
   for (int64_t g = item0 + blockIdx.x; g < item1; g += gridDim.x) {
-    int lo = t0, hi = t1 - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (tasks[mid].first_item <= g) lo = mid;
-      else hi = mid - 1;
-    }
-    const pfk_task &tk = tasks[lo];
+    const pfk_task &tk = tasks[find_task(tasks, t0, t1, g)];
     const int64_t tile = g - tk.first_item;
     const T *src = (const T *)tk.src;
     T *dst = (T *)tk.dst;
@@ -41,99 +52,173 @@ k_batch_level(const pfk_task *__restrict__ tasks, const int t0, const int t1,
   }
 }
 
+template <typename T, int NF>
+__global__ void __launch_bounds__(256)
+k_batch_small(const pfk_task *__restrict__ tasks, const int t0, const int t1,
+              const int64_t item0, const int64_t item1) {
+  extern __shared__ unsigned char smem_raw[];
+  for (int64_t g = item0 + blockIdx.x; g < item1; g += gridDim.x) {
+    const pfk_task &tk = tasks[find_task(tasks, t0, t1, g)];
+    small_tile<T, NF>(tk.pass, g - tk.first_item, (const T *)tk.src, (T *)tk.dst, smem_raw);
+    __syncthreads();
+  }
+}
+
+// kernel of a task group: 0 = general, else the short length NF
+template <typename T>
+static const void *batch_kernel(int cls) {
+  if (cls == 0) return reinterpret_cast<const void *>(&k_batch_generic<T>);
+  switch (cls) {
+#define PFK_BATCH_SMALL(N) \
+  case N: if constexpr (small_length(N, sizeof(T) == 8)) return reinterpret_cast<const void *>(&k_batch_small<T, N>); break;
+    PFK_BATCH_SMALL(2) PFK_BATCH_SMALL(3) PFK_BATCH_SMALL(4) PFK_BATCH_SMALL(5)
+    PFK_BATCH_SMALL(6) PFK_BATCH_SMALL(8) PFK_BATCH_SMALL(10) PFK_BATCH_SMALL(12)
+    PFK_BATCH_SMALL(15) PFK_BATCH_SMALL(16) PFK_BATCH_SMALL(20) PFK_BATCH_SMALL(24)
+    PFK_BATCH_SMALL(30) PFK_BATCH_SMALL(32) PFK_BATCH_SMALL(40) PFK_BATCH_SMALL(48)
+    PFK_BATCH_SMALL(60) PFK_BATCH_SMALL(64) PFK_BATCH_SMALL(80) PFK_BATCH_SMALL(96)
+    PFK_BATCH_SMALL(120) PFK_BATCH_SMALL(128)
+#undef PFK_BATCH_SMALL
+    default: break;
+  }
+  return nullptr;
+}
+
 }  // namespace pfk
 
 using namespace pfk;
 
+#define PFK_MAX_GROUPS 256
+
+typedef struct {
+  const void *kernel;
+  int t0, t1;
+  int64_t i0, i1;
+  int grid;
+  size_t lds;
+} batch_group;
+
 struct pfk_batch {
   pfk_task *d_tasks;
   int ntasks;
-  int nlev;
-  int lev_t0[PFK_MAXSTAGES + 1];       /* first task of each level (+ end) */
-  int64_t lev_i0[PFK_MAXSTAGES + 1];   /* first item of each level (+ end) */
-  size_t lds;
-  int grid;
-  int prec;
+  int ngroups;
+  batch_group groups[PFK_MAX_GROUPS];
   int use_graph;
   hipGraph_t graph;
   hipGraphExec_t exec;
 };
 
 static int batch_launch(pfk_batch *b) {
-  for (int l = 0; l < b->nlev; l++) {
-    const int64_t items = b->lev_i0[l + 1] - b->lev_i0[l];
+  for (int k = 0; k < b->ngroups; k++) {
+    batch_group *g = &b->groups[k];
+    const int64_t items = g->i1 - g->i0;
     if (items <= 0) continue;
-    const int grid = (int)(items < b->grid ? items : b->grid);
-    if (b->prec == PFK_F64)
-      hipLaunchKernelGGL(k_batch_level<double>, dim3(grid), dim3(256), b->lds, g_stream,
-                         b->d_tasks, b->lev_t0[l], b->lev_t0[l + 1], b->lev_i0[l],
-                         b->lev_i0[l + 1]);
-    else
-      hipLaunchKernelGGL(k_batch_level<float>, dim3(grid), dim3(256), b->lds, g_stream,
-                         b->d_tasks, b->lev_t0[l], b->lev_t0[l + 1], b->lev_i0[l],
-                         b->lev_i0[l + 1]);
-    hipError_t e = hipGetLastError();
+    const int grid = (int)(items < g->grid ? items : g->grid);
+    const pfk_task *tasks = b->d_tasks;
+    int t0 = g->t0, t1 = g->t1;
+    int64_t i0 = g->i0, i1 = g->i1;
+    void *args[] = {(void *)&tasks, (void *)&t0, (void *)&t1, (void *)&i0, (void *)&i1};
+    hipError_t e = hipLaunchKernel(g->kernel, dim3(grid), dim3(256), args, g->lds, g_stream);
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) { set_error("pfk_batch launch", e); return 1; }
   }
   return 0;
 }
 
+static int task_class(const pfk_task *t) {
+  if (!g_generic_only && t->pass.variant != 1 && small_covers(&t->pass)) return (int)t->pass.nf;
+  return 0;
+}
+
+static int cmp_class(const void *a, const void *b) {
+  const pfk_task *x = (const pfk_task *)a, *y = (const pfk_task *)b;
+  const int cx = task_class(x), cy = task_class(y);
+  if (cx != cy) return cx < cy ? -1 : 1;
+  return (x->pad < y->pad) ? -1 : (x->pad > y->pad);   // keep the given order otherwise
+}
+
 /* tasks must be grouped by level; task.dep marks the previous pass of the
  * same plan, so a task whose dep lies in the current level starts a new
- * level */
+ * level.  The array is reordered (by kernel within each level). */
 extern "C" pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_graph) {
   if (pfk_init() || ntasks <= 0) return nullptr;
-  pfk_batch *b = (pfk_batch *)calloc(1, sizeof(pfk_batch));
-  b->ntasks = ntasks;
-  b->prec = tasks[0].pass.prec;
-  b->use_graph = use_graph;
-  int64_t item = 0;
-  size_t lds = 0;
-  b->nlev = 0;
-  b->lev_t0[0] = 0;
-  b->lev_i0[0] = 0;
+  const int prec = tasks[0].pass.prec;
+  /* levels */
+  int lev_start[PFK_MAXSTAGES + 1], nlev = 0;
+  lev_start[0] = 0;
   for (int i = 0; i < ntasks; i++) {
-    pfk_task *t = &tasks[i];
-    if (t->pass.prec != b->prec || t->dep >= i) {
+    if (tasks[i].pass.prec != prec || tasks[i].dep >= i) {
       set_error_msg("pfk_batch: bad task list");
-      free(b);
       return nullptr;
     }
-    if (t->dep >= b->lev_t0[b->nlev]) {       /* depends on this level: next level */
-      if (b->nlev + 1 >= PFK_MAXSTAGES) { set_error_msg("pfk_batch: too many levels"); free(b); return nullptr; }
-      b->nlev++;
-      b->lev_t0[b->nlev] = i;
-      b->lev_i0[b->nlev] = item;
+    if (tasks[i].dep >= lev_start[nlev]) {
+      if (nlev + 1 >= PFK_MAXSTAGES) { set_error_msg("pfk_batch: too many levels"); return nullptr; }
+      lev_start[++nlev] = i;
     }
-    int C, pitch;
-    size_t l;
-    if (generic_tiling(&t->pass, &C, &pitch, &l)) { free(b); return nullptr; }
-    int64_t nt = (t->pass.text + C - 1) / C;
-    for (int k = 0; k < t->pass.nb; k++) nt *= t->pass.bext[k];
-    t->C = C;
-    t->pitch = pitch;
-    t->ntiles = nt;
-    t->first_item = item;
-    item += nt;
-    if (l > lds) lds = l;
+    tasks[i].pad = i;
   }
-  b->nlev++;
-  b->lev_t0[b->nlev] = ntasks;
-  b->lev_i0[b->nlev] = item;
-  b->lds = lds;
+  lev_start[++nlev] = ntasks;
+  pfk_batch *b = (pfk_batch *)calloc(1, sizeof(pfk_batch));
+  if (!b) return nullptr;
+  b->ntasks = ntasks;
+  b->use_graph = use_graph;
+  int dev = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  for (int l = 0; l < nlev; l++) {
+    const int a = lev_start[l], z = lev_start[l + 1];
+    qsort(tasks + a, (size_t)(z - a), sizeof(pfk_task), cmp_class);
+    int i = a;
+    while (i < z) {
+      const int cls = task_class(&tasks[i]);
+      int j = i;
+      if (b->ngroups >= PFK_MAX_GROUPS) {
+        set_error_msg("pfk_batch: too many task groups");
+        free(b);
+        return nullptr;
+      }
+      batch_group *g = &b->groups[b->ngroups++];
+      g->t0 = i;
+      g->i0 = 0;
+      int64_t item = 0;
+      size_t lds = 0;
+      for (; j < z && task_class(&tasks[j]) == cls; j++) {
+        pfk_task *t = &tasks[j];
+        int C = -1, pitch = 0;
+        size_t ld;
+        int64_t nt;
+        if (cls) {
+          int64_t lines = t->pass.text;
+          for (int k = 0; k < t->pass.nb; k++) lines *= t->pass.bext[k];
+          nt = (lines + 255) / 256;
+          ld = (prec == PFK_F64) ? small::lds_bytes<double>() : small::lds_bytes<float>();
+        } else {
+          if (generic_tiling(&t->pass, &C, &pitch, &ld)) { free(b); return nullptr; }
+          nt = (t->pass.text + C - 1) / C;
+          for (int k = 0; k < t->pass.nb; k++) nt *= t->pass.bext[k];
+        }
+        t->C = C;
+        t->pitch = pitch;
+        t->ntiles = nt;
+        t->first_item = item;
+        item += nt;
+        if (ld > lds) lds = ld;
+      }
+      g->t1 = j;
+      g->i1 = item;
+      g->lds = lds;
+      g->kernel = (prec == PFK_F64) ? batch_kernel<double>(cls) : batch_kernel<float>(cls);
+      if (!g->kernel) { set_error_msg("pfk_batch: no kernel"); free(b); return nullptr; }
+      int occ = 0;
+      hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, g->kernel, 256, lds);
+      if (e != hipSuccess || occ < 1) occ = 1;
+      g->grid = ncu * occ;
+      i = j;
+    }
+  }
   hipError_t e = hipMalloc(&b->d_tasks, sizeof(pfk_task) * ntasks);
   if (e == hipSuccess)
     e = hipMemcpy(b->d_tasks, tasks, sizeof(pfk_task) * ntasks, hipMemcpyHostToDevice);
   if (e != hipSuccess) { set_error("pfk_batch_create", e); pfk_batch_destroy(b); return nullptr; }
-  int dev = 0, ncu = 0, occ = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  if (b->prec == PFK_F64)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch_level<double>, 256, lds);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch_level<float>, 256, lds);
-  if (e != hipSuccess || occ < 1) occ = 1;
-  b->grid = ncu * occ;
   if (use_graph) {
     e = hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {

@@ -220,7 +220,12 @@ int pfk_launch_pass(const pfk_pass *p, const void *src, void *dst) {
     if (m == 0) return 0;
     if (m < 0) return 1;
   }
-  const int r = launch_fast(p, src, dst);   // 0: launched by a specialised kernel
+  // variant 1: general kernel; 3: short-length kernel first where it has
+  // one; else the power-of-two kernels, then the short-length kernels
+  int r = 1;
+  if (!g_generic_only && p->variant == 3) r = launch_small(p, src, dst);
+  if (r > 0) r = launch_fast(p, src, dst);
+  if (r > 0 && !g_generic_only && p->variant != 1) r = launch_small(p, src, dst);
   if (r == 0) return 0;
   if (r < 0) return 1;
   return launch_generic(p, src, dst);

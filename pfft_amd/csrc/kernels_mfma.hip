@@ -306,9 +306,14 @@ static int launch_mfma_nf(const pfk_pass *p, const void *src, void *dst) {
   return 0;
 }
 
+int mfma_covers(const pfk_pass *p) {
+  if (p->mfma == 0 || p->prec != PFK_F32 || p->kind != PFK_C2C || p->hw) return 0;
+  return p->nf >= 256 && p->nf <= 4096 && (p->nf & (p->nf - 1)) == 0;
+}
+
 // 0 launched, 1 not covered, -1 error
 int launch_mfma(const pfk_pass *p, const void *src, void *dst) {
-  if (p->mfma == 0 || p->prec != PFK_F32 || p->kind != PFK_C2C || p->hw) return 1;
+  if (!mfma_covers(p)) return 1;
   const int mode = p->mfma - 1;
   switch (p->nf) {
     case 256: return mode ? launch_mfma_nf<256, 1>(p, src, dst) : launch_mfma_nf<256, 0>(p, src, dst);

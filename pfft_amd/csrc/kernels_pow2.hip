@@ -489,8 +489,6 @@ static int launch_nf(const pfk_pass *p, const void *src, void *dst) {
   return 0;
 }
 
-static int g_cols512 = 8;
-static int g_alt1024 = 0;
 
 // Tile width C (consecutive lines per workgroup) gives 128-byte segments:
 // C = 8 for 16-byte FP64 complex, C = 16 for 8-byte FP32 complex (C = 4 / 8
@@ -502,15 +500,24 @@ static int launch_pow2_f64(const pfk_pass *p, const void *src, void *dst) {
     case 64: return launch_nf<double, 64, 8, 8>(p, src, dst);
     case 128: return launch_nf<double, 128, 8, 8>(p, src, dst);
     case 256: return launch_nf<double, 256, 8, 8>(p, src, dst);
-    case 512:
-      if (p->kind == PFK_C2C && g_cols512 == 4) return launch_nf<double, 512, 4, 8>(p, src, dst);
-      if (p->kind == PFK_C2C && g_cols512 == 16) return launch_nf<double, 512, 16, 8>(p, src, dst);
-      return launch_nf<double, 512, 8, 8>(p, src, dst);
-    case 1024:
-      if (g_alt1024) return launch_nf<double, 1024, 4, 16>(p, src, dst);
-      return launch_nf<double, 1024, 8, 16>(p, src, dst);
+    case 512: return launch_nf<double, 512, 8, 8>(p, src, dst);
+    case 1024: return launch_nf<double, 1024, 8, 16>(p, src, dst);
     case 2048: return launch_nf<double, 2048, 4, 16>(p, src, dst);
     case 4096: return launch_nf<double, 4096, 2, 16>(p, src, dst);
+    default: return 1;
+  }
+}
+
+// alternative (narrower) tiles: half the lines per workgroup, half the LDS
+static int launch_pow2_f64_alt(const pfk_pass *p, const void *src, void *dst) {
+  switch (p->nf) {
+    case 64: return launch_nf<double, 64, 4, 8>(p, src, dst);
+    case 128: return launch_nf<double, 128, 4, 8>(p, src, dst);
+    case 256: return launch_nf<double, 256, 4, 8>(p, src, dst);
+    case 512: return launch_nf<double, 512, 4, 8>(p, src, dst);
+    case 1024: return launch_nf<double, 1024, 4, 16>(p, src, dst);
+    case 2048: return launch_nf<double, 2048, 2, 16>(p, src, dst);
+    case 4096: return launch_nf<double, 4096, 1, 16>(p, src, dst);
     default: return 1;
   }
 }
@@ -521,43 +528,58 @@ static int launch_pow2_f32(const pfk_pass *p, const void *src, void *dst) {
     case 128: return launch_nf<float, 128, 16, 8>(p, src, dst);
     case 256: return launch_nf<float, 256, 16, 8>(p, src, dst);
     case 512: return launch_nf<float, 512, 16, 8>(p, src, dst);
-    case 1024:
-      if (g_alt1024) return launch_nf<float, 1024, 8, 16>(p, src, dst);
-      return launch_nf<float, 1024, 16, 16>(p, src, dst);
+    case 1024: return launch_nf<float, 1024, 16, 16>(p, src, dst);
     case 2048: return launch_nf<float, 2048, 8, 16>(p, src, dst);
     case 4096: return launch_nf<float, 4096, 4, 16>(p, src, dst);
     default: return 1;
   }
 }
 
-int launch_fast(const pfk_pass *p, const void *src, void *dst) {
-  if (g_generic_only) return 1;
-  {
-    static int init = 0;
-    if (!init) {
-      const char *e = getenv("PFFT_AMD_COLS512");
-      if (e) g_cols512 = atoi(e);
-      e = getenv("PFFT_AMD_ALT1024");
-      if (e) g_alt1024 = atoi(e);
-      init = 1;
-    }
+static int launch_pow2_f32_alt(const pfk_pass *p, const void *src, void *dst) {
+  switch (p->nf) {
+    case 64: return launch_nf<float, 64, 8, 8>(p, src, dst);
+    case 128: return launch_nf<float, 128, 8, 8>(p, src, dst);
+    case 256: return launch_nf<float, 256, 8, 8>(p, src, dst);
+    case 512: return launch_nf<float, 512, 8, 8>(p, src, dst);
+    case 1024: return launch_nf<float, 1024, 8, 16>(p, src, dst);
+    case 2048: return launch_nf<float, 2048, 4, 16>(p, src, dst);
+    case 4096: return launch_nf<float, 4096, 2, 16>(p, src, dst);
+    default: return 1;
   }
-  if (p->kind == PFK_COPY) return 1;
+}
+
+int fast_covers(const pfk_pass *p) {
+  if (p->kind == PFK_COPY) return 0;
   const int64_t nf = p->nf;
-  if (nf < 64 || nf > 4096 || (nf & (nf - 1))) return 1;
+  if (nf < 64 || nf > 4096 || (nf & (nf - 1))) return 0;
   if (p->kind == PFK_R2C) {
     // real input: contiguous, not embedded
-    if (p->n != 2 * nf || p->sin != 1 || p->zin != 0 || p->nin != p->n) return 1;
-    if (p->hw == 1) return 1;
+    if (p->n != 2 * nf || p->sin != 1 || p->zin != 0 || p->nin != p->n) return 0;
+    if (p->hw == 1) return 0;
   }
   if (p->kind == PFK_C2R) {
-    if (p->n != 2 * nf || p->sout != 1 || p->zout != 0 || p->nout != p->n) return 1;
-    if (p->hw == 2) return 1;
+    if (p->n != 2 * nf || p->sout != 1 || p->zout != 0 || p->nout != p->n) return 0;
+    if (p->hw == 2) return 0;
   }
-  if (p->kind == PFK_C2C && p->hw) return 1;
-  const int r = (p->prec == PFK_F64) ? launch_pow2_f64(p, src, dst)
-                                     : launch_pow2_f32(p, src, dst);
-  return r;   // 0 launched, 1 not covered, -1 launch error
+  if (p->kind == PFK_C2C && p->hw) return 0;
+  return 1;
+}
+
+int launch_fast(const pfk_pass *p, const void *src, void *dst) {
+  if (g_generic_only || p->variant == 1 || !fast_covers(p)) return 1;
+  if (p->variant == 2)
+    return (p->prec == PFK_F64) ? launch_pow2_f64_alt(p, src, dst)
+                                : launch_pow2_f32_alt(p, src, dst);
+  return (p->prec == PFK_F64) ? launch_pow2_f64(p, src, dst) : launch_pow2_f32(p, src, dst);
 }
 
 }  // namespace pfk
+
+extern "C" int pfk_pass_kernels(const pfk_pass *p) {
+  if (pfk::g_generic_only) return 0;
+  int m = 0;
+  if (pfk::mfma_covers(p)) m |= PFK_HAS_MFMA;
+  if (pfk::fast_covers(p)) m |= PFK_HAS_POW2;
+  if (pfk::small_covers(p)) m |= PFK_HAS_SMALL;
+  return m;
+}

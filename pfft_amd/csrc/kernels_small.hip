@@ -1,0 +1,62 @@
+// kernels_small.hip -- per-pass launch of the register-resident short-length
+// c2c kernel (small_tile.hpp): one line per lane, 256 lines per workgroup.
+#include "small_tile.hpp"
+
+namespace pfk {
+
+template <typename T, int NF>
+__global__ void __launch_bounds__(256)
+k_small(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
+        T *__restrict__ dst) {
+  extern __shared__ unsigned char smem_raw[];
+  small_tile<T, NF>(P, tile_base + blockIdx.x, src, dst, smem_raw);
+}
+
+int small_covers(const pfk_pass *p) {
+  if (p->kind != PFK_C2C || p->hw || p->nf != p->n || p->n > 128) return 0;
+  return small_length((int)p->nf, p->prec == PFK_F64) ? 1 : 0;
+}
+
+template <typename T, int NF>
+static int launch_small_nf(const pfk_pass *p, const void *src, void *dst) {
+  if constexpr (!small_length(NF, sizeof(T) == 8)) {
+    return 1;
+  } else {
+    int64_t nlines = p->text;
+    for (int i = 0; i < p->nb; ++i) nlines *= p->bext[i];
+    const int64_t ntiles = (nlines + 255) / 256;
+    const int64_t chunk = (int64_t)1 << 30;
+    for (int64_t base = 0; base < ntiles; base += chunk) {
+      const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
+      hipLaunchKernelGGL((k_small<T, NF>), dim3((unsigned)nblk), dim3(256),
+                         small::lds_bytes<T>(), g_stream, *p, base, (const T *)src, (T *)dst);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) { set_error("pfk_launch_pass(small)", e); return -1; }
+    }
+    return 0;
+  }
+}
+
+template <typename T>
+static int launch_small_t(const pfk_pass *p, const void *src, void *dst) {
+  switch (p->nf) {
+#define PFK_SMALL_LAUNCH(N) case N: return launch_small_nf<T, N>(p, src, dst);
+    PFK_SMALL_LAUNCH(2) PFK_SMALL_LAUNCH(3) PFK_SMALL_LAUNCH(4) PFK_SMALL_LAUNCH(5)
+    PFK_SMALL_LAUNCH(6) PFK_SMALL_LAUNCH(8) PFK_SMALL_LAUNCH(10) PFK_SMALL_LAUNCH(12)
+    PFK_SMALL_LAUNCH(15) PFK_SMALL_LAUNCH(16) PFK_SMALL_LAUNCH(20) PFK_SMALL_LAUNCH(24)
+    PFK_SMALL_LAUNCH(30) PFK_SMALL_LAUNCH(32) PFK_SMALL_LAUNCH(40) PFK_SMALL_LAUNCH(48)
+    PFK_SMALL_LAUNCH(60) PFK_SMALL_LAUNCH(64) PFK_SMALL_LAUNCH(80) PFK_SMALL_LAUNCH(96)
+    PFK_SMALL_LAUNCH(120) PFK_SMALL_LAUNCH(128)
+#undef PFK_SMALL_LAUNCH
+    default: return 1;
+  }
+}
+
+// 0 launched, 1 not covered, -1 error
+int launch_small(const pfk_pass *p, const void *src, void *dst) {
+  if (!small_covers(p)) return 1;
+  return (p->prec == PFK_F64) ? launch_small_t<double>(p, src, dst)
+                              : launch_small_t<float>(p, src, dst);
+}
+
+}  // namespace pfk

@@ -68,6 +68,9 @@ typedef struct pfk_pass {
   int64_t text, tsin, tsout;   /* tile dimension (text==1: none) */
   int32_t xcd_remap;  /* map consecutive tiles to one XCD (see kernels_pow2.hip) */
   int32_t mfma;       /* complex64 c2c arithmetic form: 0 plain, 1 FP32 MFMA, 2 split-BF16 MFMA */
+  int32_t variant;    /* launch variant: 0 default, 1 general kernel, 2 narrow power-of-two
+                         tiles, 3 short-length register kernel */
+  int32_t pad2;
   /* device twiddle tables (owned by the host plan):
    *   tw  : W_nf^m = exp(sign*2*pi*i*m/nf), m in [0,nf)
    *   rtw : W_n^k  = exp(sign*2*pi*i*k/n),  k in [0,n/2] (even r2c/c2r only) */
@@ -84,7 +87,7 @@ typedef struct pfk_task {
   pfk_pass pass;
   const void *src;
   void *dst;
-  int32_t C, pitch;       /* filled by pfk_batch_create */
+  int32_t C, pitch;       /* filled by pfk_batch_create (C < 0: short-length kernel) */
   int64_t ntiles;         /* filled by pfk_batch_create */
   int64_t first_item;     /* filled by pfk_batch_create */
   int32_t dep;
@@ -118,6 +121,15 @@ const char *pfk_last_error(void);
 
 /* Launch one pass on the library stream (asynchronous). */
 int  pfk_launch_pass(const pfk_pass *p, const void *src, void *dst);
+/* Kernel families with a kernel for the pass shape (bit mask):
+ * PFK_HAS_POW2 register kernels for power-of-two lengths 64..4096,
+ * PFK_HAS_SMALL one-line-per-lane kernels for short c2c lengths,
+ * PFK_HAS_MFMA MFMA kernels (complex64 arithmetic forms).  The general
+ * kernel covers every pass.  With PFFT_AMD_GENERIC_ONLY=1 the mask is 0. */
+#define PFK_HAS_POW2  1
+#define PFK_HAS_SMALL 2
+#define PFK_HAS_MFMA  4
+int  pfk_pass_kernels(const pfk_pass *p);
 
 /* Timing helpers on the library stream (HIP events). */
 int  pfk_event_create(void **ev);

@@ -26,5 +26,9 @@ void set_error_msg(const char *msg);
 int launch_fast(const pfk_pass *p, const void *src, void *dst);
 int launch_generic(const pfk_pass *p, const void *src, void *dst);
 int launch_mfma(const pfk_pass *p, const void *src, void *dst);
+int launch_small(const pfk_pass *p, const void *src, void *dst);
+int fast_covers(const pfk_pass *p);   // 1 if launch_fast has a kernel for the shape
+int mfma_covers(const pfk_pass *p);   // 1 if launch_mfma has a kernel for the shape
+int small_covers(const pfk_pass *p);  // 1 if launch_small has a kernel for the shape
 
 }  // namespace pfk

@@ -227,6 +227,33 @@ static void free_plan_memory(pa_plan *p) {
   for (int i = 0; i < 2; i++) { pfk_free(p->scratch[i]); p->scratch[i] = NULL; }
 }
 
+/* Chooses every pass's launch variant (tune.c) on temporary device
+ * buffers shaped like the plan's arrays, so planning leaves the user's
+ * arrays untouched.  Without memory for the buffers the defaults stay. */
+static void tune_plan(pa_plan *p) {
+  const unsigned effort = p->pfft_flags & (PFFT_ESTIMATE | PFFT_PATIENT | PFFT_EXHAUSTIVE);
+  int open_[PA_MAXRNK] = {0}, any = 0;
+  for (int k = 0; k < p->npass; k++) any |= open_[k] = pa_tune_lookup(&p->pass[k], effort);
+  if (!any) return;
+  void *tin = NULL, *tout = NULL;
+  const size_t ib = (size_t)(p->in_elems * (int64_t)p->in_esz);
+  const size_t ob = (size_t)(p->out_elems * (int64_t)p->out_esz);
+  if (pfk_malloc(&tin, ib) || pfk_malloc(&tout, ob) || pfk_memset(tin, 0, ib) ||
+      pfk_memset(tout, 0, ob)) {
+    pfk_free(tin);
+    pfk_free(tout);
+    return;
+  }
+  void *bufs[4] = {tin, tout, p->scratch[0], p->scratch[1]};
+  for (int k = 0; k < p->npass; k++)
+    if (open_[k] &&
+        pa_tune_measure(&p->pass[k], bufs[p->src_role[k]], bufs[p->dst_role[k]], effort))
+      break;
+  (void)pfk_sync();
+  pfk_free(tin);
+  pfk_free(tout);
+}
+
 pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const INT *no,
                         INT howmany, const INT *iblock, const INT *oblock, void *in, void *out,
                         MPI_Comm comm, int sign, const int *skip_trafos, unsigned tf,
@@ -463,6 +490,7 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
       }
     }
   }
+  tune_plan(p);
   return p;
 fail:
   free_plan_memory(p);
@@ -697,13 +725,7 @@ int pa_plan_time_passes(const pa_plan *p, void *in, void *out, int iters, double
   }
   for (int k = 0; k < p->npass; k++) {
     ms[k] /= iters;
-    const pfk_pass *ps = &p->pass[k];
-    const double rs = (p->prec == PFK_F64) ? 8.0 : 4.0;
-    const double src_sz = (ps->kind == PFK_R2C) ? rs : 2 * rs;
-    const double dst_sz = (ps->kind == PFK_C2R) ? rs : 2 * rs;
-    double nl = (double)ps->text;
-    for (int b = 0; b < ps->nb; b++) nl *= (double)ps->bext[b];
-    bytes[k] = nl * ((double)ps->nin * src_sz + (double)ps->nout * dst_sz);
+    bytes[k] = pa_pass_bytes(&p->pass[k]);
   }
   for (int k = 0; k <= p->npass; k++) pfk_event_destroy(ev[k]);
   return err;
@@ -711,7 +733,19 @@ int pa_plan_time_passes(const pa_plan *p, void *in, void *out, int iters, double
 
 int pa_plan_npasses(const pa_plan *p) { return p ? p->npass : 0; }
 
-/* One line per pass: kind, length, strides, batch and tile geometry. */
+static const char *kernel_name(const pfk_pass *ps) {
+  if (ps->kind == PFK_COPY) return "copy";
+  const int fam = pfk_pass_kernels(ps);
+  if (fam & PFK_HAS_MFMA) return ps->mfma == 1 ? "mfma-f32" : "mfma-bf16x2";
+  if (ps->variant == 1) return "general";
+  if (ps->variant == 3 && (fam & PFK_HAS_SMALL)) return "small";
+  if (fam & PFK_HAS_POW2) return ps->variant == 2 ? "fast-narrow" : "fast";
+  if (fam & PFK_HAS_SMALL) return "small";
+  return "general";
+}
+
+/* One line per pass: kind, length, strides, batch and tile geometry,
+ * the kernel that runs it. */
 int pa_plan_describe(const pa_plan *p, char *buf, size_t len) {
   static const char *kn[] = {"c2c", "r2c", "c2r", "copy"};
   size_t o = 0;
@@ -720,12 +754,13 @@ int pa_plan_describe(const pa_plan *p, char *buf, size_t len) {
     const pfk_pass *ps = &p->pass[k];
     o += (size_t)snprintf(buf + o, len - o,
                           "pass %d: %s n=%lld nf=%lld nin=%lld@%lld nout=%lld@%lld sin=%lld "
-                          "sout=%lld tile=%lld(%lld,%lld) nb=%d roles=%d->%d\n",
+                          "sout=%lld tile=%lld(%lld,%lld) nb=%d roles=%d->%d kernel=%s%s\n",
                           k, kn[ps->kind], (long long)ps->n, (long long)ps->nf,
                           (long long)ps->nin, (long long)ps->zin, (long long)ps->nout,
                           (long long)ps->zout, (long long)ps->sin, (long long)ps->sout,
                           (long long)ps->text, (long long)ps->tsin, (long long)ps->tsout, ps->nb,
-                          p->src_role[k], p->dst_role[k]);
+                          p->src_role[k], p->dst_role[k], kernel_name(ps),
+                          (ps->xcd_remap && !strncmp(kernel_name(ps), "fast", 4)) ? "+xcd" : "");
   }
   return 0;
 }

@@ -1,0 +1,330 @@
+// small_tile.hpp -- register-resident c2c pass for short mixed-radix
+// lengths (NF = 2^a 3^b 5^c <= 128 complex64, <= 64 complex128), shared by
+// the per-pass launch (kernels_small.hip) and the persistent batch kernel
+// (kernels_batch.hip).
+//
+// Work split.  A workgroup of 256 threads transforms 256 lines, one line
+// per lane: the lines of the pass are numbered flat over the tile
+// dimension and the batch dimensions, and lane `tid` of tile `b` takes line
+// 256*b + tid.  The whole line lives in registers, so the FFT stages need
+// no LDS and no barriers; every index and every twiddle index is a
+// compile-time constant after unrolling (the twiddles are wave-uniform
+// loads from the plan's exp(sign*2*pi*i*m/NF) table).
+//
+// Coalescing.  When the lines are adjacent along the tile dimension
+// (tile stride 1) lanes read element j of 64 consecutive lines together:
+// one contiguous run per wave and element.  When the line itself is
+// contiguous (element stride 1) the wave stages its 64 lines through LDS in
+// 128-byte column chunks: for each chunk the 64 lanes load 64 x 128 bytes
+// row segment by row segment (16 lanes per row for complex64), write them
+// to a [line][column] buffer padded by one slot, and each lane reads its
+// own row back.  Stores mirror the loads.
+//
+// Stages.  The Stockham stage of radix R over sub-length L maps
+//   y[(t/L)*L*R + q*L + k] = sum_r x[t + r*NF/R] W_LR^(r*k) W_R^(r*q),
+//   t in [0, NF/R), k = t mod L,
+// the recurrence of the general kernel, with the radices taken 8s first
+// (4*4 for 16), then 4, 2, 3, 5.
+#pragma once
+#include "pfk_internal.hpp"
+
+namespace pfk {
+namespace small {
+
+template <typename T>
+using C2 = cpx<T>;
+
+// radix of stage s for length NF: 8s first, then 4, 2, 3, 5
+__host__ __device__ constexpr int radix_at(int nf, int s) {
+  int r = 0;
+  for (int i = 0; i <= s; ++i) {
+    if (nf % 8 == 0 && nf != 16) r = 8;       // 16 = 4*4, not 8*2
+    else if (nf % 4 == 0) r = 4;
+    else if (nf % 2 == 0) r = 2;
+    else if (nf % 3 == 0) r = 3;
+    else if (nf % 5 == 0) r = 5;
+    else return 0;
+    nf /= r;
+  }
+  return r;
+}
+__host__ __device__ constexpr int nstages(int nf) {
+  int s = 0;
+  while (nf > 1) {
+    const int r = radix_at(nf, 0);
+    if (r == 0) return -1;
+    nf /= r;
+    ++s;
+  }
+  return s;
+}
+
+template <typename T>
+__device__ __forceinline__ C2<T> add(C2<T> a, C2<T> b) { return C2<T>{a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ C2<T> sub(C2<T> a, C2<T> b) { return C2<T>{a.x - b.x, a.y - b.y}; }
+// multiply by sg*i (sg = +-1, the transform sign)
+template <typename T>
+__device__ __forceinline__ C2<T> mul_si(C2<T> a, T sg) { return C2<T>{-sg * a.y, sg * a.x}; }
+
+// in-place DFT-R, R in {2,3,4,5,8}; W_R = exp(sg*2*pi*i/R)
+template <typename T, int R>
+__device__ __forceinline__ void dft(C2<T> *a, T sg) {
+  if constexpr (R == 2) {
+    const C2<T> s = add(a[0], a[1]), d = sub(a[0], a[1]);
+    a[0] = s;
+    a[1] = d;
+  } else if constexpr (R == 3) {
+    // W_3 = -1/2 + sg*i*sqrt(3)/2
+    const T h = (T)0.86602540378443864676372317075293618L;
+    const C2<T> b = add(a[1], a[2]), d = sub(a[1], a[2]);
+    const C2<T> t{a[0].x - (T)0.5 * b.x, a[0].y - (T)0.5 * b.y};
+    const C2<T> u = mul_si(C2<T>{h * d.x, h * d.y}, sg);
+    a[0] = add(a[0], b);
+    a[1] = add(t, u);
+    a[2] = sub(t, u);
+  } else if constexpr (R == 4) {
+    const C2<T> c0 = add(a[0], a[2]), c1 = sub(a[0], a[2]);
+    const C2<T> c2 = add(a[1], a[3]), c3 = mul_si(sub(a[1], a[3]), sg);
+    a[0] = add(c0, c2);
+    a[2] = sub(c0, c2);
+    a[1] = add(c1, c3);
+    a[3] = sub(c1, c3);
+  } else if constexpr (R == 5) {
+    // cos(2pi/5) = (sqrt5-1)/4, cos(4pi/5) = -(sqrt5+1)/4,
+    // sin(2pi/5) = sqrt((5+sqrt5)/8), sin(4pi/5) = sqrt((5-sqrt5)/8)
+    const T c1 = (T)0.30901699437494742410229341718281906L;
+    const T c2 = (T)-0.80901699437494742410229341718281906L;
+    const T s1 = (T)0.95105651629515357211643933337938214L;
+    const T s2 = (T)0.58778525229247312916870595463907277L;
+    const C2<T> b1 = add(a[1], a[4]), b2 = add(a[2], a[3]);
+    const C2<T> d1 = sub(a[1], a[4]), d2 = sub(a[2], a[3]);
+    const C2<T> t1{a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y};
+    const C2<T> t2{a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y};
+    const C2<T> u1 = mul_si(C2<T>{s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y}, sg);
+    const C2<T> u2 = mul_si(C2<T>{s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y}, sg);
+    a[0] = C2<T>{a[0].x + b1.x + b2.x, a[0].y + b1.y + b2.y};
+    a[1] = add(t1, u1);
+    a[4] = sub(t1, u1);
+    a[2] = add(t2, u2);
+    a[3] = sub(t2, u2);
+  } else if constexpr (R == 8) {
+    C2<T> e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
+    dft<T, 4>(e, sg);
+    dft<T, 4>(o, sg);
+    const T s = (T)0.70710678118654752440084436210484903928L;
+    // W8^1 = s(1 + sg i), W8^2 = sg i, W8^3 = s(-1 + sg i)
+    const C2<T> t1{s * (o[1].x - sg * o[1].y), s * (o[1].y + sg * o[1].x)};
+    const C2<T> t2 = mul_si(o[2], sg);
+    const C2<T> t3{s * (-o[3].x - sg * o[3].y), s * (-o[3].y + sg * o[3].x)};
+    a[0] = add(e[0], o[0]);
+    a[4] = sub(e[0], o[0]);
+    a[1] = add(e[1], t1);
+    a[5] = sub(e[1], t1);
+    a[2] = add(e[2], t2);
+    a[6] = sub(e[2], t2);
+    a[3] = add(e[3], t3);
+    a[7] = sub(e[3], t3);
+  }
+}
+
+// Stages S.. of the register FFT; L = product of the radices before S.
+template <typename T, int NF, int S, int L>
+__device__ __forceinline__ void fft(C2<T> *x, const C2<T> *__restrict__ tw, T sg) {
+  if constexpr (S < nstages(NF)) {
+    constexpr int R = radix_at(NF, S);
+    constexpr int NR = NF / R, LR = L * R, M = NF / LR;
+    C2<T> y[NF];
+#pragma unroll
+    for (int t = 0; t < NR; ++t) {
+      const int k = t % L;
+      C2<T> a[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        a[r] = x[t + r * NR];
+        if (k != 0 && r != 0) a[r] = cmul(a[r], tw[r * k * M]);
+      }
+      dft<T, R>(a, sg);
+#pragma unroll
+      for (int q = 0; q < R; ++q) y[(t / L) * LR + q * L + k] = a[q];
+    }
+#pragma unroll
+    for (int i = 0; i < NF; ++i) x[i] = y[i];
+    fft<T, NF, S + 1, LR>(x, tw, sg);
+  }
+}
+
+__device__ __forceinline__ int modsign(const pfk_mod &m, int64_t j) {
+  if (!m.on) return 1;
+  const int64_t t = j + m.start;
+  if (t < m.half) return ((t & 1) ? -1 : 1) * m.extra;
+  return 1;
+}
+
+// complex slots per 128-byte chunk
+template <typename T>
+constexpr int chunk() { return 128 / (int)sizeof(C2<T>); }
+
+// LDS bytes one workgroup needs
+template <typename T>
+constexpr size_t lds_bytes() {
+  return 4 * 64 * (chunk<T>() + 1) * sizeof(C2<T>) + 2 * 256 * sizeof(int64_t);
+}
+
+}  // namespace small
+
+// One tile (256 lines) of a c2c pass of length NF (= P.n).
+template <typename T, int NF>
+__device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile,
+                                           const T *__restrict__ src, T *__restrict__ dst,
+                                           unsigned char *smem) {
+  using namespace small;
+  using CT = C2<T>;
+  constexpr int K = chunk<T>();
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int64_t nlines = P.text;
+  for (int i = 0; i < P.nb; ++i) nlines *= P.bext[i];
+  const int64_t line = tile * 256 + tid;
+  const bool active = line < nlines;
+  int64_t sb = -1, db = -1;
+  if (active) {
+    const int64_t c = line % P.text;
+    int64_t bflat = line / P.text;
+    sb = c * P.tsin;
+    db = c * P.tsout;
+    for (int i = 0; i < P.nb; ++i) {
+      const int64_t e = P.bext[i];
+      const int64_t d = bflat % e;
+      bflat /= e;
+      sb += d * P.bsin[i];
+      db += d * P.bsout[i];
+    }
+  }
+  const CT *s = reinterpret_cast<const CT *>(src);
+  CT *d = reinterpret_cast<CT *>(dst);
+  CT *chunkbuf = reinterpret_cast<CT *>(smem) + wave * 64 * (K + 1);
+  int64_t *sbase = reinterpret_cast<int64_t *>(smem + 4 * 64 * (K + 1) * sizeof(CT));
+  int64_t *dbase = sbase + 256;
+  const int nin = (int)P.nin, zin = (int)P.zin, nout = (int)P.nout, zout = (int)P.zout;
+  const bool lds_in = P.sin == 1 && (P.tsin != 1 || P.text == 1);
+  const bool lds_out = P.sout == 1 && (P.tsout != 1 || P.text == 1);
+  sbase[tid] = sb;
+  dbase[tid] = db;
+  __syncthreads();
+
+  // ---------------- load (embed + input modulation) ----------------
+  CT x[NF];
+  if (lds_in) {
+#pragma unroll
+    for (int p0 = 0; p0 < NF; p0 += K) {
+      const int KC = (NF - p0) < K ? (NF - p0) : K;   // columns in this chunk
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        if (i < KC) {
+          const int e = lane + 64 * i;
+          const int row = e / KC, col = e - row * KC;
+          const int64_t rb = sbase[wave * 64 + row];
+          const int j = p0 + col - zin;
+          CT v{(T)0, (T)0};
+          if (rb >= 0 && j >= 0 && j < nin) {
+            v = s[rb + j];
+            const T f = (T)modsign(P.imod, j);
+            v = CT{v.x * f, v.y * f};
+          }
+          chunkbuf[row * (K + 1) + col] = v;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < K; ++q)
+        if (q < KC) x[(p0 + q) < NF ? p0 + q : NF - 1] = chunkbuf[lane * (K + 1) + q];
+      __syncthreads();
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < NF; ++p) {
+      const int j = p - zin;
+      CT v{(T)0, (T)0};
+      if (active && j >= 0 && j < nin) {
+        v = s[sb + (int64_t)j * P.sin];
+        const T f = (T)modsign(P.imod, j);
+        v = CT{v.x * f, v.y * f};
+      }
+      x[p] = v;
+    }
+  }
+
+  // ---------------- FFT in registers ----------------
+  const T sg = (T)P.sign;
+  fft<T, NF, 0, 1>(x, reinterpret_cast<const CT *>(P.tw), sg);
+
+  // ---------------- store (trunc + output modulation) ----------------
+  if (lds_out) {
+#pragma unroll
+    for (int p0 = 0; p0 < NF; p0 += K) {
+      const int KC = (NF - p0) < K ? (NF - p0) : K;
+#pragma unroll
+      for (int q = 0; q < K; ++q)
+        if (q < KC) chunkbuf[lane * (K + 1) + q] = x[(p0 + q) < NF ? p0 + q : NF - 1];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        if (i < KC) {
+          const int e = lane + 64 * i;
+          const int row = e / KC, col = e - row * KC;
+          const int64_t rb = dbase[wave * 64 + row];
+          const int k = p0 + col - zout;
+          if (rb >= 0 && k >= 0 && k < nout) {
+            CT v = chunkbuf[row * (K + 1) + col];
+            const T f = (T)modsign(P.omod, k);
+            d[rb + k] = CT{v.x * f, v.y * f};
+          }
+        }
+      }
+      __syncthreads();
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < NF; ++p) {
+      const int k = p - zout;
+      if (active && k >= 0 && k < nout) {
+        const T f = (T)modsign(P.omod, k);
+        d[db + (int64_t)k * P.sout] = CT{x[p].x * f, x[p].y * f};
+      }
+    }
+  }
+}
+
+// Lengths with a small_tile instantiation, per precision.
+__host__ __device__ constexpr bool small_length(int nf, bool f64) {
+  switch (nf) {
+    case 2: case 3: case 4: case 5: case 6: case 8: case 10: case 12: case 15: case 16:
+    case 20: case 24: case 30: case 32: case 40: case 48: case 60: case 64:
+      return true;
+    case 80: case 96: case 120: case 128:
+      return !f64;
+    default:
+      return false;
+  }
+}
+
+// Runs small_tile for the pass's length (one of small_length's).
+template <typename T>
+__device__ __forceinline__ void small_dispatch(const pfk_pass &P, const int64_t tile,
+                                               const T *src, T *dst, unsigned char *smem) {
+  constexpr bool F64 = sizeof(T) == 8;
+  switch (P.nf) {
+#define PFK_SMALL_CASE(N) \
+  case N: if constexpr (small_length(N, F64)) small_tile<T, N>(P, tile, src, dst, smem); break;
+    PFK_SMALL_CASE(2) PFK_SMALL_CASE(3) PFK_SMALL_CASE(4) PFK_SMALL_CASE(5) PFK_SMALL_CASE(6)
+    PFK_SMALL_CASE(8) PFK_SMALL_CASE(10) PFK_SMALL_CASE(12) PFK_SMALL_CASE(15)
+    PFK_SMALL_CASE(16) PFK_SMALL_CASE(20) PFK_SMALL_CASE(24) PFK_SMALL_CASE(30)
+    PFK_SMALL_CASE(32) PFK_SMALL_CASE(40) PFK_SMALL_CASE(48) PFK_SMALL_CASE(60)
+    PFK_SMALL_CASE(64) PFK_SMALL_CASE(80) PFK_SMALL_CASE(96) PFK_SMALL_CASE(120)
+    PFK_SMALL_CASE(128)
+#undef PFK_SMALL_CASE
+    default: break;
+  }
+}
+
+}  // namespace pfk

@@ -30,15 +30,20 @@ def random_input(spec, seed=0):
     return rng.uniform(-1.0, 1.0, n_scalars(spec, "in"))
 
 
-def run_gpu(spec, x, prec="f64", comm=None, adjoint=False, plan_flags_extra=0):
-    """Execute `spec` on the GPU; x is the flat input (float64 scalars)."""
+def run_gpu(spec, x, prec="f64", comm=None, adjoint=False, plan_flags_extra=0, inplace=False):
+    """Execute `spec` on the GPU; x is the flat input (float64 scalars).
+    inplace: one array serves as input and output (sized for the larger)."""
     rdt = np.float64 if prec == "f64" else np.float32
     esz = np.dtype(rdt).itemsize
     nin, nout = n_scalars(spec, "in"), n_scalars(spec, "out")
     if adjoint:
         nin, nout = nout, nin
-    din = P.DeviceBuffer(max(nin, 1) * esz)
-    dout = P.DeviceBuffer(max(nout, 1) * esz)
+    if inplace:
+        din = P.DeviceBuffer(max(nin, nout, 1) * esz)
+        dout = din
+    else:
+        din = P.DeviceBuffer(max(nin, 1) * esz)
+        dout = P.DeviceBuffer(max(nout, 1) * esz)
     if comm is None:
         comm = comm_for(spec.rnk_pm)
     skip = spec.skip if any(spec.skip) else None
@@ -53,7 +58,8 @@ def run_gpu(spec, x, prec="f64", comm=None, adjoint=False, plan_flags_extra=0):
     out = dout.download(rdt, nout).astype(np.float64)
     plan.destroy()
     din.free()
-    dout.free()
+    if not inplace:
+        dout.free()
     return out
 
 
