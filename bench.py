@@ -46,17 +46,25 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 counter summary (profiles/), or None."""
-    p = os.path.join(ROOT, "profiles", "traffic_r2c_c2r_512.json")
-    if not os.path.exists(p):
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "r2c_c2r_512_fp64_hbm_traffic.md")
+
+
+def traffic_from_profiles(plan, npass):
+    """HBM bytes per launch of pass `npass` of plan `plan` ("r2c"/"c2r")
+    from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by the same counters on a copy kernel
+    of known size), or None when the summary has no such row."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
         return None
-    try:
-        with open(p) as f:
-            return json.load(f)
-    except Exception:
-        return None
+    with open(TRAFFIC_SUMMARY) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 7 and cells[0] == plan and cells[1] == str(npass):
+                try:
+                    return int(float(cells[6]))
+                except ValueError:
+                    return None
+    return None
 
 
 def cpu_baseline(n, cores, seed):
@@ -150,14 +158,14 @@ def main():
     sum_ms = sum(p["ms"] for p in passes)
     bytes_pair = sum(p["bytes"] for p in passes)
 
-    traffic = traffic_from_profiles()
+    traffic = traffic_from_profiles(dom["plan"], dom["pass"])
     roofline = {
         "bound": "hbm",
         "achieved": round(dom["GBps"], 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4),
-        "traffic": (traffic or {}).get("bytes_per_launch") if traffic else None,
+        "traffic": traffic,
         "kernel": "%s pass %d (%s)" % (dom["plan"], dom["pass"], dom["desc"]),
         "algorithmic_bytes_per_launch": dom["bytes"],
         "avg_launch_ms": round(dom["ms"], 4),

@@ -38,6 +38,25 @@ def _time(fn, reps, warm=1):
     return (time.perf_counter() - t0) * 1e3 / reps
 
 
+def _pass_times(plan, in_ptr, out_ptr, iters=5):
+    """Per-pass mean duration (HIP events on the library stream), bytes
+    moved, bandwidth and the kernel that runs each pass."""
+    L = P.lib()
+    L.pfft_amd_time_passes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]
+    L.pfft_amd_plan_npasses.argtypes = [ctypes.c_void_p]
+    npass = L.pfft_amd_plan_npasses(plan.handle)
+    ms = (ctypes.c_double * npass)()
+    by = (ctypes.c_double * npass)()
+    if L.pfft_amd_time_passes(plan.handle, in_ptr, out_ptr, iters, ms, by):
+        return None
+    lines = plan.describe()
+    return [{"ms": round(ms[k], 4), "GBps": round(by[k] / (ms[k] * 1e-3) / 1e9, 1),
+             "kernel": lines[k].split("kernel=")[-1],
+             "n": int(lines[k].split(" n=")[1].split()[0])} for k in range(npass)]
+
+
 def cfg3(args):
     import torch
     dev = torch.device("cuda:0")
@@ -72,6 +91,7 @@ def cfg3(args):
     ms64 = _time(lambda: (fw.execute(), bw.execute()), 3)
     flops = 2 * 5.0 * N * math.log2(N)
     res["forms"].append({"form": "fp64 plain", "ms_fwd_bwd": round(ms64, 3),
+                         "passes_fwd": _pass_times(fw, x64.data_ptr(), y64.data_ptr()),
                          "GFLOPs": round(flops / (ms64 * 1e-3) / 1e9, 1),
                          "rel_l2_fwd": 0.0, "max_err_fwd": 0.0,
                          "roundtrip_rel_l2": float((z64 / N - x64).norm() / x64.norm())})
@@ -89,11 +109,12 @@ def cfg3(args):
             res["forms"].append({"form": form, "status": "plan not available"})
             continue
         ms = _time(lambda: (fw.execute(), bw.execute()), 3)
+        passes = _pass_times(fw, x.data_ptr(), y.data_ptr())
         fw.execute()
         d = (y.to(torch.complex128) - ref)
         bw.execute()
         res["forms"].append({
-            "form": form, "ms_fwd_bwd": round(ms, 3),
+            "form": form, "ms_fwd_bwd": round(ms, 3), "passes_fwd": passes,
             "GFLOPs": round(flops / (ms * 1e-3) / 1e9, 1),
             "speedup_vs_fp64": round(ms64 / ms, 3),
             "rel_l2_fwd": float(d.norm().item() / refn),

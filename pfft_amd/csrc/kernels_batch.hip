@@ -48,7 +48,7 @@ k_batch_generic(const pfk_task *__restrict__ tasks, const int t0, const int t1,
       case PFK_C2R: pass_tile<T, PFK_C2R, 16>(tk.pass, tk.C, tk.pitch, tile, src, dst, smem_raw); break;
       default: pass_tile<T, PFK_COPY, 16>(tk.pass, tk.C, tk.pitch, tile, src, dst, smem_raw); break;
     }
-    __syncthreads();   // LDS is reused by the next item
+    lds_barrier();   // LDS is reused by the next item
   }
 }
 
@@ -60,7 +60,7 @@ k_batch_small(const pfk_task *__restrict__ tasks, const int t0, const int t1,
   for (int64_t g = item0 + blockIdx.x; g < item1; g += gridDim.x) {
     const pfk_task &tk = tasks[find_task(tasks, t0, t1, g)];
     small_tile<T, NF>(tk.pass, g - tk.first_item, (const T *)tk.src, (T *)tk.dst, smem_raw);
-    __syncthreads();
+    lds_barrier();
   }
 }
 

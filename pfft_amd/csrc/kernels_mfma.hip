@@ -38,11 +38,12 @@ struct f2 {
 
 __device__ __forceinline__ f2 f2mul(f2 a, f2 b) { return f2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
-__device__ __forceinline__ int msign_m(const pfk_mod &m, int64_t j) {
-  if (!m.on) return 1;
+// +-1 modulation factor of index j, with selects only (no branch between a
+// load and its use)
+__device__ __forceinline__ float msign_m(const pfk_mod &m, int64_t j) {
   const int64_t t = j + m.start;
-  if (t < m.half) return ((t & 1) ? -1 : 1) * m.extra;
-  return 1;
+  const int s = (m.on && t < m.half) ? ((t & 1) ? -m.extra : m.extra) : 1;
+  return (float)s;
 }
 
 // round-to-nearest-even float -> bf16 bits
@@ -101,19 +102,28 @@ k_c2c_mfma(const pfk_pass P, const int64_t tile_base, const float *__restrict__ 
   const bool st_tile = (P.tsout == 1 && P.sout != 1);
 
   // ---- load the tile into LDS (embed + input modulation) ----
-  for (int e = tid; e < NF * C; e += NT) {
-    int c, p;
-    if (ld_tile) { c = e % C; p = e / C; }
-    else { p = e % NF; c = e / NF; }
-    const int j = p - zin;
-    f2 v{0.f, 0.f};
-    if (c < nl && j >= 0 && j < nin) {
-      v = s[sb + (int64_t)c * P.tsin + (int64_t)j * P.sin];
-      const float f = (float)msign_m(P.imod, j);
-      v.x *= f;
-      v.y *= f;
+  {
+    // all loads of the thread are issued before any is used; positions
+    // outside the data read the tile's first element and are dropped
+    constexpr int PERL = NF * C / NT;
+    f2 raw[PERL];
+#pragma unroll
+    for (int i = 0; i < PERL; ++i) {
+      const int e = tid + i * NT;
+      const int c = ld_tile ? e % C : e / NF, p = ld_tile ? e / C : e % NF;
+      const int j = p - zin;
+      const bool ok = c < nl && j >= 0 && j < nin;
+      raw[i] = s[ok ? sb + (int64_t)c * P.tsin + (int64_t)j * P.sin : sb];
     }
-    lds[p * C + c] = v;
+#pragma unroll
+    for (int i = 0; i < PERL; ++i) {
+      const int e = tid + i * NT;
+      const int c = ld_tile ? e % C : e / NF, p = ld_tile ? e / C : e % NF;
+      const int j = p - zin;
+      const bool ok = c < nl && j >= 0 && j < nin;
+      const float f = msign_m(P.imod, j);
+      lds[p * C + c] = ok ? f2{raw[i].x * f, raw[i].y * f} : f2{0.f, 0.f};
+    }
   }
 
   // ---- DFT-16 matrix operands held by this lane ----
@@ -146,7 +156,7 @@ k_c2c_mfma(const pfk_pass P, const int64_t tile_base, const float *__restrict__ 
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- radix-16 MFMA stages ----
   constexpr int NBF = NF / 16;               // butterflies per line per stage
@@ -209,7 +219,7 @@ k_c2c_mfma(const pfk_pass P, const int64_t tile_base, const float *__restrict__ 
         yi[w][v] = acci[v];
       }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int w = 0; w < TPW; ++w) {
       const int tile = wave * TPW + w;
@@ -222,7 +232,7 @@ k_c2c_mfma(const pfk_pass P, const int64_t tile_base, const float *__restrict__ 
         lds[((b - k) * 16 + k + q * L) * C + c] = f2{yr[w][v], yi[w][v]};
       }
     }
-    __syncthreads();
+    lds_barrier();
     L *= 16;
   }
 
@@ -258,7 +268,7 @@ k_c2c_mfma(const pfk_pass P, const int64_t tile_base, const float *__restrict__ 
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int g = tid + u * NT;
@@ -269,7 +279,7 @@ k_c2c_mfma(const pfk_pass P, const int64_t tile_base, const float *__restrict__ 
         for (int q = 0; q < RL; ++q) lds[((b - k) * RL + k + q * L) * C + c] = res[u][q];
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
 
   // ---- store (trunc + output modulation + undo the per-stage scaling) ----
@@ -280,7 +290,7 @@ k_c2c_mfma(const pfk_pass P, const int64_t tile_base, const float *__restrict__ 
     else { k = e % nout; c = e / nout; }
     if (c >= nl) continue;
     f2 v = lds[(k + zout) * C + c];
-    const float f = (float)msign_m(P.omod, k) * unscale;
+    const float f = msign_m(P.omod, k) * unscale;
     d[db + (int64_t)c * P.tsout + (int64_t)k * P.sout] = f2{v.x * f, v.y * f};
   }
 }

@@ -150,11 +150,14 @@ __device__ __forceinline__ void dft16(C2<T> *v) {
   for (int k = 0; k < 16; ++k) v[k] = o[k];
 }
 
-__device__ __forceinline__ int msign(const pfk_mod &m, int64_t j) {
-  if (!m.on) return 1;
+// +-1 modulation factor of index j, computed with selects only: a branch
+// here would sit between a load and its use and make the compiler wait for
+// every load before issuing the next one.
+template <typename T>
+__device__ __forceinline__ T msign(const pfk_mod &m, int64_t j) {
   const int64_t t = j + m.start;
-  if (t < m.half) return ((t & 1) ? -1 : 1) * m.extra;
-  return 1;
+  const int s = (m.on && t < m.half) ? ((t & 1) ? -m.extra : m.extra) : 1;
+  return (T)s;
 }
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
@@ -202,10 +205,10 @@ __device__ __forceinline__ void exchange(C2<T> *v, const int *pos, C2<T> *lds, c
   constexpr int TT = NF / E;
 #pragma unroll
   for (int m = 0; m < E; ++m) lds[pos[m] * C + c] = v[m];
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = lds[(t + TT * m) * C + c];
-  __syncthreads();
+  lds_barrier();
 }
 
 // Full length-NF FFT on registers; first stage input = v (positions t+T*m),
@@ -242,17 +245,19 @@ struct TileCtx {
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  With remap on, XCD x runs a contiguous range of tiles in order, so
 // tiles that share cache lines at their edges run on the same L2 close in
-// time.  The map is a bijection on [0, nblk) for any nblk.
-__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nblk) {
-  const int64_t q = nblk >> 3, r = nblk & 7;
+// time.  The map is a bijection on [0, n) for any n; item i runs on XCD
+// i mod 8 because the persistent grid is a multiple of 8.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n) {
+  const int64_t q = n >> 3, r = n & 7;
   const int64_t x = b & 7, j = b >> 3;
   return x * q + (x < r ? x : r) + j;
 }
 
-__device__ __forceinline__ TileCtx tile_of(const pfk_pass &P, int C, int64_t tile_base) {
+// Tile of work item `item` (0 <= item < ntiles).
+__device__ __forceinline__ TileCtx tile_at(const pfk_pass &P, int C, int64_t item,
+                                           int64_t ntiles) {
   const int64_t ntt = (P.text + C - 1) / C;
-  const int64_t tileid =
-      tile_base + (P.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x);
+  const int64_t tileid = P.xcd_remap ? xcd_remap(item, ntiles) : item;
   const int64_t tt = tileid % ntt;
   int64_t bflat = tileid / ntt;
   const int64_t c0 = tt * C;
@@ -270,223 +275,378 @@ __device__ __forceinline__ TileCtx tile_of(const pfk_pass &P, int C, int64_t til
   return r;
 }
 
-// ------------------------------------------------------------------ c2c
-template <typename T, int NF, int C, int E, int SIGN>
-__global__ void __launch_bounds__(C *NF / E)
-k_c2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
-           T *__restrict__ dst) {
+// Persistent pass kernels.  A fixed grid of resident workgroups walks the
+// tiles (workgroup b takes items item0 + b, + G, + 2G, ... below item_end);
+// launched with one workgroup per tile instead (G = item_end - item0) the
+// same kernel does one tile per workgroup.  While a tile is
+// transformed and stored, the loads of the workgroup's next tile are
+// already in flight in registers (software pipelining across tiles), so
+// HBM stays busy through the compute and LDS phases.  On the last item the
+// prefetch degenerates to every lane reading the tile's first element (one
+// cached line), which keeps the load sequence free of branches.
+//
+// Loads are issued unconditionally from clamped in-bounds addresses and
+// masked afterwards: a branch between a load and its use would make the
+// compiler wait for each load in turn.
+
+// Twiddle tables live in LDS when they fit next to the exchange buffer
+// (limit 144 KiB per workgroup): LDS reads count on lgkmcnt, so a twiddle
+// read does not wait for the prefetched tile the way a global load would
+// (vmcnt completes in order).
+template <typename T>
+constexpr bool tw_in_lds(int nf, int c, int extra) {
+  return ((size_t)nf * c + nf + extra) * sizeof(C2<T>) <= 144 * 1024;
+}
+template <typename T>
+__device__ __forceinline__ void stage_table(C2<T> *dst, const C2<T> *src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+// c2c: raw loads of elements t + T*m (minus the embed offset) of line c
+template <typename T, int NF, int C, int E>
+__device__ __forceinline__ void load_c2c(const pfk_pass &P, const T *src, const TileCtx &tc,
+                                         const bool live, const int c, const int t,
+                                         C2<T> *v) {
   constexpr int TT = NF / E;
-  __shared__ C2<T> lds[NF * C];
-  const int tid = threadIdx.x;
-  const int c = tid % C, t = tid / C;
-  const TileCtx tc = tile_of(P, C, tile_base);
-  const bool valid = c < tc.nl;
-  const C2<T> *s = reinterpret_cast<const C2<T> *>(src) + tc.sb + (int64_t)c * P.tsin;
-  C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
-  const C2<T> *tw = reinterpret_cast<const C2<T> *>(P.tw);
-  const int zin = (int)P.zin, nin = (int)P.nin, zout = (int)P.zout, nout = (int)P.nout;
-  C2<T> v[E];
+  const int zin = (int)P.zin, nin = (int)P.nin;
+  const C2<T> *s0 = reinterpret_cast<const C2<T> *>(src) + tc.sb;
+  const C2<T> *s = (live && c < tc.nl) ? s0 + (int64_t)c * P.tsin : s0;
+  const int64_t sin = live ? P.sin : 0;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int j = t + TT * m - zin;
-    C2<T> x{0, 0};
-    if (valid && j >= 0 && j < nin) {
-      x = ld_stream(s + (int64_t)j * P.sin);
-      const T f = (T)msign(P.imod, j);
-      x.x *= f;
-      x.y *= f;
-    }
-    v[m] = x;
+    const bool ok = j >= 0 && j < nin;
+    v[m] = ld_stream(s + (int64_t)(ok ? j : 0) * sin);
   }
-  fft_regs<T, NF, C, E, SIGN>(v, lds, t, c, tw);
-  if (valid) {
+}
+
+template <typename T, int NF, int C, int E, int SIGN>
+__global__ void __launch_bounds__(C *NF / E)
+k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
+           const T *__restrict__ src, T *__restrict__ dst) {
+  constexpr int TT = NF / E;
+  constexpr bool TWL = tw_in_lds<T>(NF, C, 0);
+  __shared__ C2<T> lds[NF * C];
+  __shared__ C2<T> twl[TWL ? NF : 1];
+  const int tid = threadIdx.x;
+  const int c0 = tid % C, t0 = tid / C;
+  const C2<T> *twg = reinterpret_cast<const C2<T> *>(P.tw);
+  if constexpr (TWL) stage_table(twl, twg, NF);
+  const C2<T> *tw = TWL ? twl : twg;
+  const int zin = (int)P.zin, nin = (int)P.nin, zout = (int)P.zout, nout = (int)P.nout;
+  int64_t item = item0 + blockIdx.x;
+  TileCtx tc = tile_at(P, C, item, ntiles);
+  // one step: prefetch the next tile into `nxt`, transform and store `cur`;
+  // false after the last item.  Two steps per loop trip alternate the
+  // buffers, so no register copy has to wait for the prefetch.
+  auto step = [&](C2<T> *cur, C2<T> *nxt) -> bool {
+    // lane indices made opaque per step: index arithmetic is recomputed in
+    // the loop instead of being hoisted and kept live in registers
+    int t = t0, c = c0;
+    asm volatile("" : "+v"(t), "+v"(c));
+    const int64_t ni = item + gridDim.x;
+    const bool more = ni < item_end;
+    const TileCtx tn = tile_at(P, C, more ? ni : item, ntiles);
+    load_c2c<T, NF, C, E>(P, src, tn, more, c, t, nxt);
+    const bool valid = c < tc.nl;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const int k = t + TT * m - zout;
-      if (k >= 0 && k < nout) {
-        const T f = (T)msign(P.omod, k);
-        st_stream(d + (int64_t)k * P.sout, C2<T>{v[m].x * f, v[m].y * f});
+      const int j = t + TT * m - zin;
+      const bool ok = valid && j >= 0 && j < nin;
+      const T f = msign<T>(P.imod, j);
+      cur[m] = ok ? C2<T>{cur[m].x * f, cur[m].y * f} : C2<T>{0, 0};
+    }
+    fft_regs<T, NF, C, E, SIGN>(cur, lds, t, c, tw);
+    if (valid) {
+      C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + TT * m - zout;
+        if (k >= 0 && k < nout) {
+          const T f = msign<T>(P.omod, k);
+          st_stream(d + (int64_t)k * P.sout, C2<T>{cur[m].x * f, cur[m].y * f});
+        }
       }
     }
+    if (!more) return false;
+    lds_barrier();   // the next tile reuses the LDS exchange buffer
+    item = ni;
+    tc = tn;
+    return true;
+  };
+  C2<T> va[E], vb[E];
+  load_c2c<T, NF, C, E>(P, src, tc, true, c0, t0, va);
+  if constexpr (TWL) lds_barrier();
+  while (step(va, vb) && step(vb, va)) {
   }
 }
 
 // ------------------------------------------------------------------ r2c
 // real line of n = 2*NF contiguous samples (sin == 1, no embed) ->
 // kept half-spectrum outputs [zout, zout+nout) of the n/2+1.
+template <typename T, int NF, int C, int E>
+__device__ __forceinline__ void load_r2c(const pfk_pass &P, const T *src, const TileCtx &tc,
+                                         const bool live, const int c, const int t,
+                                         C2<T> *v) {
+  constexpr int TT = NF / E;
+  const T *s0 = src + tc.sb;
+  const C2<T> *s = reinterpret_cast<const C2<T> *>((live && c < tc.nl) ? s0 + (int64_t)c * P.tsin
+                                                                     : s0);
+  const int step = live ? TT : 0;
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = ld_stream(s + (live ? t : 0) + step * m);
+}
+
 template <typename T, int NF, int C, int E, int SIGN>
 __global__ void __launch_bounds__(C *NF / E)
-k_r2c_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
-           T *__restrict__ dst) {
+k_r2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
+           const T *__restrict__ src, T *__restrict__ dst) {
   constexpr int TT = NF / E;
   constexpr int H = NF;  // h = n/2
+  constexpr bool TWL = tw_in_lds<T>(NF, C, NF);
   __shared__ C2<T> lds[NF * C];
+  __shared__ C2<T> twl[TWL ? NF : 1], rtwl[TWL ? NF : 1];
   const int tid = threadIdx.x;
-  const int c = tid % C, t = tid / C;
-  const TileCtx tc = tile_of(P, C, tile_base);
-  const bool valid = c < tc.nl;
-  const C2<T> *s = reinterpret_cast<const C2<T> *>(src + tc.sb + (int64_t)c * P.tsin);
-  C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
-  const C2<T> *tw = reinterpret_cast<const C2<T> *>(P.tw);
-  const C2<T> *rtw = reinterpret_cast<const C2<T> *>(P.rtw);
-  C2<T> v[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int p = t + TT * m;
-    C2<T> x{0, 0};
-    if (valid) {
-      x = ld_stream(s + p);
-      if (P.imod.on) {
-        x.x *= (T)msign(P.imod, 2 * p);
-        x.y *= (T)msign(P.imod, 2 * p + 1);
-      }
-    }
-    v[m] = x;
+  const int c0 = tid % C, t0 = tid / C;
+  const C2<T> *twg = reinterpret_cast<const C2<T> *>(P.tw);
+  const C2<T> *rtwg = reinterpret_cast<const C2<T> *>(P.rtw);
+  if constexpr (TWL) {
+    stage_table(twl, twg, NF);
+    stage_table(rtwl, rtwg, NF);
   }
-  fft_regs<T, NF, C, E, SIGN>(v, lds, t, c, tw);
-  // mirror exchange: Z_{h-k}
-#pragma unroll
-  for (int m = 0; m < E; ++m) lds[(t + TT * m) * C + c] = v[m];
-  __syncthreads();
+  const C2<T> *tw = TWL ? twl : twg;
+  const C2<T> *rtw = TWL ? rtwl : rtwg;
   const int zout = (int)P.zout, nout = (int)P.nout;
-  if (valid) {
+  int64_t item = item0 + blockIdx.x;
+  TileCtx tc = tile_at(P, C, item, ntiles);
+  auto step = [&](C2<T> *cur, C2<T> *nxt) -> bool {
+    int t = t0, c = c0;
+    asm volatile("" : "+v"(t), "+v"(c));
+    const int64_t ni = item + gridDim.x;
+    const bool more = ni < item_end;
+    const TileCtx tn = tile_at(P, C, more ? ni : item, ntiles);
+    load_r2c<T, NF, C, E>(P, src, tn, more, c, t, nxt);
+    const bool valid = c < tc.nl;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const int k = t + TT * m;
-      const C2<T> a = v[m];
-      const C2<T> b = lds[((H - k) & (H - 1)) * C + c];
-      // Ev = (a + conj b)/2, O = -i (a - conj b)/2, X = Ev + W^k O
-      const C2<T> Ev{(a.x + b.x) * (T)0.5, (a.y - b.y) * (T)0.5};
-      const C2<T> O{(a.y + b.y) * (T)0.5, -(a.x - b.x) * (T)0.5};
-      const int ko = k - zout;
-      if (ko >= 0 && ko < nout) {
-        T f = (T)msign(P.omod, ko);
-        if (P.hw == 2 && k != 0) f *= (T)2;
-        const C2<T> WO = cmul(O, rtw[k]);
-        st_stream(d + (int64_t)ko * P.sout, C2<T>{(Ev.x + WO.x) * f, (Ev.y + WO.y) * f});
-      }
-      if (k == 0) {  // Nyquist X_h = E_0 - O_0 (W^h = -1)
-        const int kn = H - zout;
-        if (kn >= 0 && kn < nout) {
-          const T f = (T)msign(P.omod, kn);
-          d[(int64_t)kn * P.sout] = C2<T>{(Ev.x - O.x) * f, (Ev.y - O.y) * f};
+      const int p = t + TT * m;
+      const T fx = msign<T>(P.imod, 2 * p), fy = msign<T>(P.imod, 2 * p + 1);
+      cur[m] = valid ? C2<T>{cur[m].x * fx, cur[m].y * fy} : C2<T>{0, 0};
+    }
+    fft_regs<T, NF, C, E, SIGN>(cur, lds, t, c, tw);
+    // mirror exchange: Z_{h-k}
+#pragma unroll
+    for (int m = 0; m < E; ++m) lds[(t + TT * m) * C + c] = cur[m];
+    lds_barrier();
+    if (valid) {
+      C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + TT * m;
+        const C2<T> a = cur[m];
+        const C2<T> b = lds[((H - k) & (H - 1)) * C + c];
+        // Ev = (a + conj b)/2, O = -i (a - conj b)/2, X = Ev + W^k O
+        const C2<T> Ev{(a.x + b.x) * (T)0.5, (a.y - b.y) * (T)0.5};
+        const C2<T> O{(a.y + b.y) * (T)0.5, -(a.x - b.x) * (T)0.5};
+        const int ko = k - zout;
+        if (ko >= 0 && ko < nout) {
+          T f = msign<T>(P.omod, ko);
+          if (P.hw == 2 && k != 0) f *= (T)2;
+          const C2<T> WO = cmul(O, rtw[k]);
+          st_stream(d + (int64_t)ko * P.sout, C2<T>{(Ev.x + WO.x) * f, (Ev.y + WO.y) * f});
+        }
+        if (k == 0) {  // Nyquist X_h = E_0 - O_0 (W^h = -1)
+          const int kn = H - zout;
+          if (kn >= 0 && kn < nout) {
+            const T f = msign<T>(P.omod, kn);
+            d[(int64_t)kn * P.sout] = C2<T>{(Ev.x - O.x) * f, (Ev.y - O.y) * f};
+          }
         }
       }
     }
+    if (!more) return false;
+    lds_barrier();
+    item = ni;
+    tc = tn;
+    return true;
+  };
+  C2<T> va[E], vb[E];
+  load_r2c<T, NF, C, E>(P, src, tc, true, c0, t0, va);
+  if constexpr (TWL) lds_barrier();
+  while (step(va, vb) && step(vb, va)) {
   }
 }
 
 // ------------------------------------------------------------------ c2r
 // half spectrum (embedded at zin) -> real line of n = 2*NF contiguous
 // samples (sout == 1, no trunc).
+template <typename T, int NF, int C, int E>
+__device__ __forceinline__ void load_c2r(const pfk_pass &P, const T *src, const TileCtx &tc,
+                                         const bool live, const int c, const int t, C2<T> *v,
+                                         C2<T> &xh) {
+  load_c2c<T, NF, C, E>(P, src, tc, live, c, t, v);
+  const int jh = NF - (int)P.zin;
+  const bool okh = jh >= 0 && jh < (int)P.nin;
+  const C2<T> *s0 = reinterpret_cast<const C2<T> *>(src) + tc.sb;
+  const C2<T> *s = (live && c < tc.nl) ? s0 + (int64_t)c * P.tsin : s0;
+  xh = s[(int64_t)((live && okh) ? jh : 0) * P.sin];
+}
+
 template <typename T, int NF, int C, int E, int SIGN>
 __global__ void __launch_bounds__(C *NF / E)
-k_c2r_pow2(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
-           T *__restrict__ dst) {
+k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
+           const T *__restrict__ src, T *__restrict__ dst) {
   constexpr int TT = NF / E;
   constexpr int H = NF;
+  constexpr bool TWL = tw_in_lds<T>(NF, C, C + NF);
   __shared__ C2<T> lds[(NF + 1) * C];
+  __shared__ C2<T> twl[TWL ? NF : 1], rtwl[TWL ? NF : 1];
   const int tid = threadIdx.x;
-  const int c = tid % C, t = tid / C;
-  const TileCtx tc = tile_of(P, C, tile_base);
-  const bool valid = c < tc.nl;
-  const C2<T> *s = reinterpret_cast<const C2<T> *>(src) + tc.sb + (int64_t)c * P.tsin;
-  C2<T> *d = reinterpret_cast<C2<T> *>(dst + tc.db + (int64_t)c * P.tsout);
-  const C2<T> *tw = reinterpret_cast<const C2<T> *>(P.tw);
-  const C2<T> *rtw = reinterpret_cast<const C2<T> *>(P.rtw);
+  const int c0 = tid % C, t0 = tid / C;
+  const C2<T> *twg = reinterpret_cast<const C2<T> *>(P.tw);
+  const C2<T> *rtwg = reinterpret_cast<const C2<T> *>(P.rtw);
+  if constexpr (TWL) {
+    stage_table(twl, twg, NF);
+    stage_table(rtwl, rtwg, NF);
+  }
+  const C2<T> *tw = TWL ? twl : twg;
+  const C2<T> *rtw = TWL ? rtwl : rtwg;
   const int zin = (int)P.zin, nin = (int)P.nin;
-  // load X_k, k = t + T*m, and X_h (thread t == 0) into LDS
-  C2<T> v[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int k = t + TT * m;
-    const int j = k - zin;
-    C2<T> x{0, 0};
-    if (valid && j >= 0 && j < nin) {
-      x = ld_stream(s + (int64_t)j * P.sin);
-      T f = (T)msign(P.imod, j);
-      if (P.hw == 1 && k != 0) f *= (T)0.5;
-      x.x *= f;
-      x.y *= f;
-    }
-    v[m] = x;
-    lds[k * C + c] = x;
-  }
-  if (t == 0) {
-    const int j = H - zin;
-    C2<T> x{0, 0};
-    if (valid && j >= 0 && j < nin) {
-      x = s[(int64_t)j * P.sin];
-      x.x *= (T)msign(P.imod, j);
-    }
-    lds[H * C + c] = x;
-  }
-  __syncthreads();
-  // Z_k = E_k + i O_k
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int k = t + TT * m;
-    const C2<T> a = v[m];
-    C2<T> z;
-    if (k == 0) {
-      const T xh = lds[H * C + c].x;
-      z = C2<T>{a.x + xh, a.x - xh};
-    } else {
-      const C2<T> b = lds[(H - k) * C + c];
-      const C2<T> Ev{a.x + b.x, a.y - b.y};
-      const C2<T> O = cmul(C2<T>{a.x - b.x, a.y + b.y}, rtw[k]);
-      z = C2<T>{Ev.x - O.y, Ev.y + O.x};
-    }
-    v[m] = z;
-  }
-  __syncthreads();
-  fft_regs<T, NF, C, E, SIGN>(v, lds, t, c, tw);
-  if (valid) {
+  int64_t item = item0 + blockIdx.x;
+  TileCtx tc = tile_at(P, C, item, ntiles);
+  auto step = [&](C2<T> *cur, C2<T> &curh, C2<T> *nxt, C2<T> &nxth) -> bool {
+    int t = t0, c = c0;
+    asm volatile("" : "+v"(t), "+v"(c));
+    const int64_t ni = item + gridDim.x;
+    const bool more = ni < item_end;
+    const TileCtx tn = tile_at(P, C, more ? ni : item, ntiles);
+    load_c2r<T, NF, C, E>(P, src, tn, more, c, t, nxt, nxth);
+    const bool valid = c < tc.nl;
+    // X_k, k = t + T*m, and X_h (kept by thread t == 0) into LDS
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const int p = t + TT * m;
-      C2<T> x = v[m];
-      if (P.omod.on) {
-        x.x *= (T)msign(P.omod, 2 * p);
-        x.y *= (T)msign(P.omod, 2 * p + 1);
-      }
-      st_stream(d + p, x);
+      const int k = t + TT * m;
+      const int j = k - zin;
+      const bool ok = valid && j >= 0 && j < nin;
+      T f = msign<T>(P.imod, j);
+      if (P.hw == 1 && k != 0) f *= (T)0.5;
+      cur[m] = ok ? C2<T>{cur[m].x * f, cur[m].y * f} : C2<T>{0, 0};
+      lds[k * C + c] = cur[m];
     }
+    if (t == 0) {
+      const int jh = H - zin;
+      const bool okh = valid && jh >= 0 && jh < nin;
+      lds[H * C + c] = okh ? C2<T>{curh.x * msign<T>(P.imod, jh), (T)0} : C2<T>{0, 0};
+    }
+    lds_barrier();
+    // Z_k = E_k + i O_k
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = t + TT * m;
+      const C2<T> a = cur[m];
+      C2<T> z;
+      if (k == 0) {
+        const T xhr = lds[H * C + c].x;
+        z = C2<T>{a.x + xhr, a.x - xhr};
+      } else {
+        const C2<T> b = lds[(H - k) * C + c];
+        const C2<T> Ev{a.x + b.x, a.y - b.y};
+        const C2<T> O = cmul(C2<T>{a.x - b.x, a.y + b.y}, rtw[k]);
+        z = C2<T>{Ev.x - O.y, Ev.y + O.x};
+      }
+      cur[m] = z;
+    }
+    lds_barrier();
+    fft_regs<T, NF, C, E, SIGN>(cur, lds, t, c, tw);
+    if (valid) {
+      C2<T> *d = reinterpret_cast<C2<T> *>(dst + tc.db + (int64_t)c * P.tsout);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int p = t + TT * m;
+        C2<T> x = cur[m];
+        if (P.omod.on) {
+          x.x *= msign<T>(P.omod, 2 * p);
+          x.y *= msign<T>(P.omod, 2 * p + 1);
+        }
+        st_stream(d + p, x);
+      }
+    }
+    if (!more) return false;
+    lds_barrier();
+    item = ni;
+    tc = tn;
+    return true;
+  };
+  C2<T> va[E], vb[E], ha, hb;
+  load_c2r<T, NF, C, E>(P, src, tc, true, c0, t0, va, ha);
+  if constexpr (TWL) lds_barrier();
+  while (step(va, ha, vb, hb) && step(vb, hb, va, ha)) {
   }
 }
 
 // ------------------------------------------------------------------ dispatch
+// Persistent grid: resident workgroups per CU (occupancy of the kernel at
+// its LDS size) times the CU count, a multiple of 8 (the XCD count).
+template <typename K>
+static int64_t persistent_grid(K kernel, int threads) {
+  int dev = 0, ncu = 0, occ = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess ||
+      occ < 1)
+    occ = 1;
+  if (ncu < 8) ncu = 8;
+  return (int64_t)occ * (ncu / 8) * 8;
+}
+
+// Launches kernel k over `ntiles` items: persistent (grid = resident
+// workgroups, `pgrid`) or one workgroup per tile (in chunks of 2^30).
+template <typename T>
+static int launch_items(void (*k)(pfk_pass, int64_t, int64_t, int64_t, const T *, T *),
+                        const pfk_pass *p, int64_t ntiles, int64_t pgrid, bool persistent,
+                        int threads, const T *src, T *dst) {
+  if (persistent) {
+    const int64_t g = ntiles < pgrid ? ntiles : pgrid;
+    hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(threads), 0, g_stream, *p, (int64_t)0, ntiles,
+                       ntiles, src, dst);
+  } else {
+    const int64_t chunk = (int64_t)1 << 30;
+    for (int64_t base = 0; base < ntiles; base += chunk) {
+      const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
+      hipLaunchKernelGGL(k, dim3((unsigned)nblk), dim3(threads), 0, g_stream, *p, base,
+                         base + nblk, ntiles, src, dst);
+    }
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_error("pfk_launch_pass(pow2)", e); return -1; }
+  return 0;
+}
+
+template <typename T, int NF, int C, int E, int SIGN>
+static int launch_kind(const pfk_pass *p, int64_t ntiles, const void *src, void *dst) {
+  constexpr int NT = C * NF / E;
+  static int64_t grid_c2c = 0, grid_r2c = 0, grid_c2r = 0;   // per instantiation
+  const bool persistent = !(p->variant & 4);
+  const T *s = (const T *)src;
+  T *d = (T *)dst;
+  if (p->kind == PFK_C2C) {
+    if (!grid_c2c) grid_c2c = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN>, NT);
+    return launch_items(k_c2c_pow2<T, NF, C, E, SIGN>, p, ntiles, grid_c2c, persistent, NT, s, d);
+  } else if (p->kind == PFK_R2C) {
+    if (!grid_r2c) grid_r2c = persistent_grid(k_r2c_pow2<T, NF, C, E, SIGN>, NT);
+    return launch_items(k_r2c_pow2<T, NF, C, E, SIGN>, p, ntiles, grid_r2c, persistent, NT, s, d);
+  }
+  if (!grid_c2r) grid_c2r = persistent_grid(k_c2r_pow2<T, NF, C, E, SIGN>, NT);
+  return launch_items(k_c2r_pow2<T, NF, C, E, SIGN>, p, ntiles, grid_c2r, persistent, NT, s, d);
+}
+
 template <typename T, int NF, int C, int E>
 static int launch_nf(const pfk_pass *p, const void *src, void *dst) {
   int64_t ntiles = (p->text + C - 1) / C;
   for (int i = 0; i < p->nb; ++i) ntiles *= p->bext[i];
   if (ntiles <= 0) return 0;
-  const T *s = (const T *)src;
-  T *d = (T *)dst;
-  const int64_t chunk = (int64_t)1 << 30;
-  for (int64_t base = 0; base < ntiles; base += chunk) {
-    const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
-    dim3 grid((unsigned)nblk), block(C * NF / E);
-    if (p->kind == PFK_C2C) {
-      if (p->sign < 0)
-        hipLaunchKernelGGL((k_c2c_pow2<T, NF, C, E, -1>), grid, block, 0, g_stream, *p, base, s, d);
-      else
-        hipLaunchKernelGGL((k_c2c_pow2<T, NF, C, E, +1>), grid, block, 0, g_stream, *p, base, s, d);
-    } else if (p->kind == PFK_R2C) {
-      if (p->sign < 0)
-        hipLaunchKernelGGL((k_r2c_pow2<T, NF, C, E, -1>), grid, block, 0, g_stream, *p, base, s, d);
-      else
-        hipLaunchKernelGGL((k_r2c_pow2<T, NF, C, E, +1>), grid, block, 0, g_stream, *p, base, s, d);
-    } else {
-      if (p->sign < 0)
-        hipLaunchKernelGGL((k_c2r_pow2<T, NF, C, E, -1>), grid, block, 0, g_stream, *p, base, s, d);
-      else
-        hipLaunchKernelGGL((k_c2r_pow2<T, NF, C, E, +1>), grid, block, 0, g_stream, *p, base, s, d);
-    }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) { set_error("pfk_launch_pass(pow2)", e); return -1; }
-  }
-  return 0;
+  return p->sign < 0 ? launch_kind<T, NF, C, E, -1>(p, ntiles, src, dst)
+                     : launch_kind<T, NF, C, E, +1>(p, ntiles, src, dst);
 }
 
 
@@ -567,7 +727,7 @@ int fast_covers(const pfk_pass *p) {
 
 int launch_fast(const pfk_pass *p, const void *src, void *dst) {
   if (g_generic_only || p->variant == 1 || !fast_covers(p)) return 1;
-  if (p->variant == 2)
+  if ((p->variant & 3) == 2)
     return (p->prec == PFK_F64) ? launch_pow2_f64_alt(p, src, dst)
                                 : launch_pow2_f32_alt(p, src, dst);
   return (p->prec == PFK_F64) ? launch_pow2_f64(p, src, dst) : launch_pow2_f32(p, src, dst);

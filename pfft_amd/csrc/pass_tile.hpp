@@ -6,11 +6,32 @@
 
 namespace pfk {
 
+// +-1 modulation factor of index j, with selects only (a branch between a
+// load and its use makes the compiler wait for each load in turn)
 __device__ __forceinline__ int mod_sign(const pfk_mod &m, int64_t j) {
-  if (!m.on) return 1;
   const int64_t t = j + m.start;
-  if (t < m.half) return ((t & 1) ? -1 : 1) * m.extra;
-  return 1;
+  return (m.on && t < m.half) ? ((t & 1) ? -m.extra : m.extra) : 1;
+}
+
+// Runs st(i, ld(i)) for i = tid, tid + NT, ... < total, U indices at a
+// time with the U loads issued before any is used.  ld(i) must be safe for
+// any i in [0, total).
+template <int U, typename V, typename Ld, typename St>
+__device__ __forceinline__ void batched(const int tid, const int NT, const int total, Ld ld,
+                                        St st) {
+  for (int b = tid; b < total; b += U * NT) {
+    V v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = b + u * NT;
+      v[u] = ld(i < total ? i : b);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = b + u * NT;
+      if (i < total) st(i, v[u]);
+    }
+  }
 }
 
 // One tile (C lines) of a pass: the whole pass computation for the lines
@@ -53,14 +74,19 @@ __device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const 
         const int cc = idx / n, p = idx - cc * n;
         if (p < zin || p >= zin + nin) buf[cc * pitch + p] = C2{0, 0};
       }
-    for (int idx = tid; idx < nl * nin; idx += NT) {
-      int cc, j;
-      if (ld_tile_fast) { cc = idx % nl; j = idx / nl; }
-      else { j = idx % nin; cc = idx / nin; }
-      C2 v = s[sb + cc * P.tsin + (int64_t)j * P.sin];
-      const T f = (T)mod_sign(P.imod, j);
-      buf[cc * pitch + zin + j] = C2{v.x * f, v.y * f};
-    }
+    batched<8, C2>(
+        tid, NT, nl * nin,
+        [&](int idx) {
+          const int cc = ld_tile_fast ? idx % nl : idx / nin;
+          const int j = ld_tile_fast ? idx / nl : idx % nin;
+          return s[sb + cc * P.tsin + (int64_t)j * P.sin];
+        },
+        [&](int idx, C2 v) {
+          const int cc = ld_tile_fast ? idx % nl : idx / nin;
+          const int j = ld_tile_fast ? idx / nl : idx % nin;
+          const T f = (T)mod_sign(P.imod, j);
+          buf[cc * pitch + zin + j] = C2{v.x * f, v.y * f};
+        });
   } else if (KIND == PFK_R2C) {
     if ((n & 1) == 0) {
       // real slot p of line cc is rbuf[2*cc*pitch + p]: complex slot p/2
@@ -68,25 +94,35 @@ __device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const 
         const int cc = idx / n, p = idx - cc * n;
         if (p < zin || p >= zin + nin) rbuf[2 * cc * pitch + p] = (T)0;
       }
-      for (int idx = tid; idx < nl * nin; idx += NT) {
-        int cc, j;
-        if (ld_tile_fast) { cc = idx % nl; j = idx / nl; }
-        else { j = idx % nin; cc = idx / nin; }
-        const T v = src[sb + cc * P.tsin + (int64_t)j * P.sin];
-        rbuf[2 * cc * pitch + zin + j] = v * (T)mod_sign(P.imod, j);
-      }
+      batched<8, T>(
+          tid, NT, nl * nin,
+          [&](int idx) {
+            const int cc = ld_tile_fast ? idx % nl : idx / nin;
+            const int j = ld_tile_fast ? idx / nl : idx % nin;
+            return src[sb + cc * P.tsin + (int64_t)j * P.sin];
+          },
+          [&](int idx, T v) {
+            const int cc = ld_tile_fast ? idx % nl : idx / nin;
+            const int j = ld_tile_fast ? idx / nl : idx % nin;
+            rbuf[2 * cc * pitch + zin + j] = v * (T)mod_sign(P.imod, j);
+          });
     } else {
       for (int idx = tid; idx < nl * n; idx += NT) {
         const int cc = idx / n, p = idx - cc * n;
         if (p < zin || p >= zin + nin) buf[cc * pitch + p] = C2{0, 0};
       }
-      for (int idx = tid; idx < nl * nin; idx += NT) {
-        int cc, j;
-        if (ld_tile_fast) { cc = idx % nl; j = idx / nl; }
-        else { j = idx % nin; cc = idx / nin; }
-        const T v = src[sb + cc * P.tsin + (int64_t)j * P.sin];
-        buf[cc * pitch + zin + j] = C2{v * (T)mod_sign(P.imod, j), (T)0};
-      }
+      batched<8, T>(
+          tid, NT, nl * nin,
+          [&](int idx) {
+            const int cc = ld_tile_fast ? idx % nl : idx / nin;
+            const int j = ld_tile_fast ? idx / nl : idx % nin;
+            return src[sb + cc * P.tsin + (int64_t)j * P.sin];
+          },
+          [&](int idx, T v) {
+            const int cc = ld_tile_fast ? idx % nl : idx / nin;
+            const int j = ld_tile_fast ? idx / nl : idx % nin;
+            buf[cc * pitch + zin + j] = C2{v * (T)mod_sign(P.imod, j), (T)0};
+          });
     }
   } else {  // PFK_C2R
     const C2 *s = reinterpret_cast<const C2 *>(src);
@@ -97,40 +133,50 @@ __device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const 
           const int cc = idx / (h + 1), p = idx - cc * (h + 1);
           if (p < zin || p >= zin + nin) buf[cc * pitch + p] = C2{0, 0};
         }
-      for (int idx = tid; idx < nl * nin; idx += NT) {
-        int cc, j;
-        if (ld_tile_fast) { cc = idx % nl; j = idx / nl; }
-        else { j = idx % nin; cc = idx / nin; }
-        C2 v = s[sb + cc * P.tsin + (int64_t)j * P.sin];
-        T f = (T)mod_sign(P.imod, j);
-        const int p = zin + j;
-        if (P.hw == 1 && p != 0 && p != h) f *= (T)0.5;
-        buf[cc * pitch + p] = C2{v.x * f, v.y * f};
-      }
+      batched<8, C2>(
+          tid, NT, nl * nin,
+          [&](int idx) {
+            const int cc = ld_tile_fast ? idx % nl : idx / nin;
+            const int j = ld_tile_fast ? idx / nl : idx % nin;
+            return s[sb + cc * P.tsin + (int64_t)j * P.sin];
+          },
+          [&](int idx, C2 v) {
+            const int cc = ld_tile_fast ? idx % nl : idx / nin;
+            const int j = ld_tile_fast ? idx / nl : idx % nin;
+            T f = (T)mod_sign(P.imod, j);
+            const int p = zin + j;
+            if (P.hw == 1 && p != 0 && p != h) f *= (T)0.5;
+            buf[cc * pitch + p] = C2{v.x * f, v.y * f};
+          });
     } else {
       for (int idx = tid; idx < nl * n; idx += NT) {
         const int cc = idx / n, p = idx - cc * n;
         buf[cc * pitch + p] = C2{0, 0};
       }
-      __syncthreads();
-      for (int idx = tid; idx < nl * nin; idx += NT) {
-        int cc, j;
-        if (ld_tile_fast) { cc = idx % nl; j = idx / nl; }
-        else { j = idx % nin; cc = idx / nin; }
-        C2 v = s[sb + cc * P.tsin + (int64_t)j * P.sin];
-        T f = (T)mod_sign(P.imod, j);
-        const int p = zin + j;
-        if (P.hw == 1 && p != 0) f *= (T)0.5;
-        if (p == 0) {
-          buf[cc * pitch] = C2{v.x * f, (T)0};
-        } else {
-          buf[cc * pitch + p] = C2{v.x * f, v.y * f};
-          buf[cc * pitch + n - p] = C2{v.x * f, -v.y * f};
-        }
-      }
+      lds_barrier();
+      batched<8, C2>(
+          tid, NT, nl * nin,
+          [&](int idx) {
+            const int cc = ld_tile_fast ? idx % nl : idx / nin;
+            const int j = ld_tile_fast ? idx / nl : idx % nin;
+            return s[sb + cc * P.tsin + (int64_t)j * P.sin];
+          },
+          [&](int idx, C2 v) {
+            const int cc = ld_tile_fast ? idx % nl : idx / nin;
+            const int j = ld_tile_fast ? idx / nl : idx % nin;
+            T f = (T)mod_sign(P.imod, j);
+            const int p = zin + j;
+            if (P.hw == 1 && p != 0) f *= (T)0.5;
+            if (p == 0) {
+              buf[cc * pitch] = C2{v.x * f, (T)0};
+            } else {
+              buf[cc * pitch + p] = C2{v.x * f, v.y * f};
+              buf[cc * pitch + n - p] = C2{v.x * f, -v.y * f};
+            }
+          });
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   const C2 *tw = reinterpret_cast<const C2 *>(P.tw);
   const C2 *rtw = reinterpret_cast<const C2 *>(P.rtw);
@@ -160,7 +206,7 @@ __device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const 
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
 
   // ---------------- FFT stages (output-centric Stockham) ----------------
@@ -191,7 +237,7 @@ __device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const 
           res[e] = acc;
         }
       }
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int e = 0; e < EPT; ++e) {
         const int og = tid + e * NT;
@@ -200,7 +246,7 @@ __device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const 
           buf[cc * pitch + o] = res[e];
         }
       }
-      __syncthreads();
+      lds_barrier();
       L = LR;
     }
   }

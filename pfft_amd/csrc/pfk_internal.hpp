@@ -15,6 +15,16 @@ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
   return cpx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
+// Workgroup barrier ordering LDS accesses only.  The pass kernels share
+// nothing through global memory inside a launch, so a barrier need not wait
+// for outstanding global stores (or prefetched loads, which complete in
+// order with them) the way __syncthreads() does.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 extern hipStream_t g_stream;
 extern int g_generic_only;   // PFFT_AMD_GENERIC_ONLY=1: route every pass to the general kernel
 extern int g_max_lds;

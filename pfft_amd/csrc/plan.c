@@ -739,7 +739,10 @@ static const char *kernel_name(const pfk_pass *ps) {
   if (fam & PFK_HAS_MFMA) return ps->mfma == 1 ? "mfma-f32" : "mfma-bf16x2";
   if (ps->variant == 1) return "general";
   if (ps->variant == 3 && (fam & PFK_HAS_SMALL)) return "small";
-  if (fam & PFK_HAS_POW2) return ps->variant == 2 ? "fast-narrow" : "fast";
+  if (fam & PFK_HAS_POW2) {
+    static const char *nm[4] = {"fast", "fast-narrow", "fast-grid", "fast-narrow-grid"};
+    return nm[((ps->variant & 3) == 2 ? 1 : 0) + ((ps->variant & 4) ? 2 : 0)];
+  }
   if (fam & PFK_HAS_SMALL) return "small";
   return "general";
 }

@@ -154,11 +154,13 @@ __device__ __forceinline__ void fft(C2<T> *x, const C2<T> *__restrict__ tw, T sg
   }
 }
 
-__device__ __forceinline__ int modsign(const pfk_mod &m, int64_t j) {
-  if (!m.on) return 1;
+// +-1 modulation factor of index j, with selects only (a branch between a
+// load and its use makes the compiler wait for each load in turn)
+template <typename T>
+__device__ __forceinline__ T modsign(const pfk_mod &m, int64_t j) {
   const int64_t t = j + m.start;
-  if (t < m.half) return ((t & 1) ? -1 : 1) * m.extra;
-  return 1;
+  const int s = (m.on && t < m.half) ? ((t & 1) ? -m.extra : m.extra) : 1;
+  return (T)s;
 }
 
 // complex slots per 128-byte chunk
@@ -210,7 +212,7 @@ __device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile
   const bool lds_out = P.sout == 1 && (P.tsout != 1 || P.text == 1);
   sbase[tid] = sb;
   dbase[tid] = db;
-  __syncthreads();
+  lds_barrier();
 
   // ---------------- load (embed + input modulation) ----------------
   CT x[NF];
@@ -218,6 +220,18 @@ __device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile
 #pragma unroll
     for (int p0 = 0; p0 < NF; p0 += K) {
       const int KC = (NF - p0) < K ? (NF - p0) : K;   // columns in this chunk
+      CT v[K];
+#pragma unroll
+      for (int i = 0; i < K; ++i) {   // issue every load of the chunk first
+        if (i < KC) {
+          const int e = lane + 64 * i;
+          const int row = e / KC, col = e - row * KC;
+          const int64_t rb = sbase[wave * 64 + row];
+          const int j = p0 + col - zin;
+          const bool ok = rb >= 0 && j >= 0 && j < nin;
+          v[i] = s[ok ? rb + j : 0];
+        }
+      }
 #pragma unroll
       for (int i = 0; i < K; ++i) {
         if (i < KC) {
@@ -225,32 +239,31 @@ __device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile
           const int row = e / KC, col = e - row * KC;
           const int64_t rb = sbase[wave * 64 + row];
           const int j = p0 + col - zin;
-          CT v{(T)0, (T)0};
-          if (rb >= 0 && j >= 0 && j < nin) {
-            v = s[rb + j];
-            const T f = (T)modsign(P.imod, j);
-            v = CT{v.x * f, v.y * f};
-          }
-          chunkbuf[row * (K + 1) + col] = v;
+          const bool ok = rb >= 0 && j >= 0 && j < nin;
+          const T f = modsign<T>(P.imod, j);
+          chunkbuf[row * (K + 1) + col] = ok ? CT{v[i].x * f, v[i].y * f} : CT{(T)0, (T)0};
         }
       }
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int q = 0; q < K; ++q)
         if (q < KC) x[(p0 + q) < NF ? p0 + q : NF - 1] = chunkbuf[lane * (K + 1) + q];
-      __syncthreads();
+      lds_barrier();
     }
   } else {
+    const int64_t sbv = active ? sb : 0;   // inactive lanes read element 0 and drop it
 #pragma unroll
     for (int p = 0; p < NF; ++p) {
       const int j = p - zin;
-      CT v{(T)0, (T)0};
-      if (active && j >= 0 && j < nin) {
-        v = s[sb + (int64_t)j * P.sin];
-        const T f = (T)modsign(P.imod, j);
-        v = CT{v.x * f, v.y * f};
-      }
-      x[p] = v;
+      const bool ok = j >= 0 && j < nin;
+      x[p] = s[sbv + (int64_t)(ok ? j : 0) * P.sin];
+    }
+#pragma unroll
+    for (int p = 0; p < NF; ++p) {
+      const int j = p - zin;
+      const bool ok = active && j >= 0 && j < nin;
+      const T f = modsign<T>(P.imod, j);
+      x[p] = ok ? CT{x[p].x * f, x[p].y * f} : CT{(T)0, (T)0};
     }
   }
 
@@ -266,7 +279,7 @@ __device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile
 #pragma unroll
       for (int q = 0; q < K; ++q)
         if (q < KC) chunkbuf[lane * (K + 1) + q] = x[(p0 + q) < NF ? p0 + q : NF - 1];
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int i = 0; i < K; ++i) {
         if (i < KC) {
@@ -276,19 +289,19 @@ __device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile
           const int k = p0 + col - zout;
           if (rb >= 0 && k >= 0 && k < nout) {
             CT v = chunkbuf[row * (K + 1) + col];
-            const T f = (T)modsign(P.omod, k);
+            const T f = modsign<T>(P.omod, k);
             d[rb + k] = CT{v.x * f, v.y * f};
           }
         }
       }
-      __syncthreads();
+      lds_barrier();
     }
   } else {
 #pragma unroll
     for (int p = 0; p < NF; ++p) {
       const int k = p - zout;
       if (active && k >= 0 && k < nout) {
-        const T f = (T)modsign(P.omod, k);
+        const T f = modsign<T>(P.omod, k);
         d[db + (int64_t)k * P.sout] = CT{x[p].x * f, x[p].y * f};
       }
     }

@@ -15,6 +15,8 @@
  *   (0,0) power-of-two register kernel, default tile width;
  *   (0,1) the same with the XCD-aware workgroup order;
  *   (2,0) (2,1) power-of-two register kernel, half the tile width;
+ *   (4,r) (6,r) as (0,r) (2,r) with one workgroup per tile instead of
+ *         the persistent grid;
  *   (3,0) short-length kernel, one line per lane;
  *   (1,0) the general kernel.
  * A pass with one candidate is not timed.  All candidates compute the
@@ -124,7 +126,7 @@ int pa_wisdom_import(const char *filename) {
     int v, r;
     double ms;
     if (!fgets(line, sizeof line, f) || sscanf(line, "%319s %d %d %lf", key, &v, &r, &ms) != 4 ||
-        v < 0 || v > 3 || r < 0 || r > 1) {
+        v < 0 || v > 6 || v == 5 || r < 0 || r > 1) {
       ok = 0;
       break;
     }
@@ -212,7 +214,8 @@ int pa_tune_measure(pfk_pass *ps, const void *src, void *dst, unsigned effort) {
   const char *rv = getenv("PFFT_AMD_XCD_REMAP");
   const int remap_fixed = (rv && *rv) ? (rv[0] == '1') : -1;
   const int reps = (effort & PFFT_EXHAUSTIVE) ? 10 : (effort & PFFT_PATIENT) ? 6 : 3;
-  static const int cand[6][2] = {{0, 0}, {0, 1}, {2, 0}, {2, 1}, {3, 0}, {1, 0}};
+  static const int cand[10][2] = {{0, 0}, {0, 1}, {2, 0}, {2, 1}, {4, 0},
+                                  {4, 1}, {6, 0}, {6, 1}, {3, 0}, {1, 0}};
   const int fam = pfk_pass_kernels(ps);
   void *e0 = NULL, *e1 = NULL;
   if (pfk_event_create(&e0) || pfk_event_create(&e1)) {
@@ -222,11 +225,12 @@ int pa_tune_measure(pfk_pass *ps, const void *src, void *dst, unsigned effort) {
   const int v0 = ps->variant, r0 = ps->xcd_remap;
   int best_v = v0, best_r = r0, err = 0;
   double best = -1;
-  for (int c = 0; c < 6 && !err; c++) {
+  for (int c = 0; c < 10 && !err; c++) {
     const int v = cand[c][0], r = cand[c][1];
-    if ((v == 0 || v == 2) && !(fam & PFK_HAS_POW2)) continue;
+    const int pow2 = v == 0 || v == 2 || v == 4 || v == 6;
+    if (pow2 && !(fam & PFK_HAS_POW2)) continue;
     if (v == 3 && !(fam & PFK_HAS_SMALL)) continue;
-    if ((v == 0 || v == 2) && remap_fixed >= 0 && r != remap_fixed) continue;
+    if (pow2 && remap_fixed >= 0 && r != remap_fixed) continue;
     ps->variant = v;
     ps->xcd_remap = r;
     const double t = time_launches(ps, src, dst, reps, e0, e1);

@@ -74,11 +74,13 @@ def _expected(line, prec, variant, remap):
     if variant == 3 and small:
         return "small"
     if pow2:
-        return ("fast-narrow" if variant == 2 else "fast") + ("+xcd" if remap else "")
+        name = {0: "fast", 2: "fast-narrow", 3: "fast", 4: "fast-grid", 6: "fast-narrow-grid"}
+        return name[variant] + ("+xcd" if remap else "")
     return "small" if small else "general"
 
 
-@pytest.mark.parametrize("variant,remap", [(0, 0), (0, 1), (2, 0), (2, 1), (3, 0), (1, 0)])
+@pytest.mark.parametrize("variant,remap", [(0, 0), (0, 1), (2, 0), (2, 1), (4, 0), (4, 1),
+                                           (6, 0), (6, 1), (3, 0), (1, 0)])
 @pytest.mark.parametrize("kind,n,flags,prec", CASES)
 def test_forced_variant_parity(tune_all, tmp_path, kind, n, flags, prec, variant, remap):
     spec, lines = _plan_lines(kind, n, flags, prec)     # tunes, fills the cache
@@ -107,9 +109,9 @@ def test_measure_fills_cache_and_reuses(tune_all):
     t0 = P.tune_timings()
     _plan_lines("c2c", [64, 128, 256], TO, "f64")
     n1, t1 = P.wisdom_count(), P.tune_timings()
-    # three passes; candidates per pass: four power-of-two variants, the
+    # three passes; candidates per pass: eight power-of-two variants, the
     # short-length kernel where the length has one (64 in FP64), the general one
-    assert n1 == 3 and t1 - t0 == 6 + 5 + 5
+    assert n1 == 3 and t1 - t0 == 10 + 9 + 9
     _plan_lines("c2c", [64, 128, 256], TO, "f64")
     assert P.wisdom_count() == n1 and P.tune_timings() == t1
     # different geometry -> new selections

@@ -39,7 +39,7 @@ def test_import_export_roundtrip(tmp_path):
 @pytest.mark.parametrize("text", [
     "not-wisdom 1 1\nk 0 0 1.0\n",
     "pfft-amd-wisdom 1 2\nkey 0 0 1.0\n",             # fewer lines than announced
-    "pfft-amd-wisdom 1 1\nkey 4 0 1.0\n",             # unknown variant
+    "pfft-amd-wisdom 1 1\nkey 5 0 1.0\n",             # unknown variant
     "pfft-amd-wisdom 1 1\nkey 0 2 1.0\n",             # remap is 0 or 1
     "pfft-amd-wisdom 1 1\nkey zero 0 1.0\n",
 ])
