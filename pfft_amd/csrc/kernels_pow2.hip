@@ -198,23 +198,37 @@ __device__ __forceinline__ void stage(C2<T> *v, int *pos, const int t,
   for (int m = 0; m < E; ++m) v[m] = o[m];
 }
 
+// LDS slot of position p of line c.  Tile-major lane order (consecutive
+// lanes = consecutive lines): [position][line], conflict-free without
+// padding.  Line-major lane order (consecutive lanes = consecutive
+// positions of one line): [line][position] with one slot of padding per 16
+// positions, so the radix-8/16 output scatters (position strides 8 and 16)
+// spread over the banks.
+template <int NF, int C, bool LM>
+__device__ __forceinline__ int lidx(int p, int c) {
+  if constexpr (LM) return c * (NF + NF / 16) + p + (p >> 4);
+  else return p * C + c;
+}
+template <int NF, int C, bool LM>
+constexpr int lds_slots() { return LM ? C * (NF + NF / 16) : NF * C; }
+
 // Exchange through LDS: write v[m] at position pos[m], read back t + T*m.
-template <typename T, int NF, int C, int E>
+template <typename T, int NF, int C, int E, bool LM>
 __device__ __forceinline__ void exchange(C2<T> *v, const int *pos, C2<T> *lds, const int t,
                                          const int c) {
   constexpr int TT = NF / E;
 #pragma unroll
-  for (int m = 0; m < E; ++m) lds[pos[m] * C + c] = v[m];
+  for (int m = 0; m < E; ++m) lds[lidx<NF, C, LM>(pos[m], c)] = v[m];
   lds_barrier();
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = lds[(t + TT * m) * C + c];
+  for (int m = 0; m < E; ++m) v[m] = lds[lidx<NF, C, LM>(t + TT * m, c)];
   lds_barrier();
 }
 
 // Full length-NF FFT on registers; first stage input = v (positions t+T*m),
 // output X[t + T*m] in v[m].  Stages of radix E, then one of radix
 // NF / E^A if NF is not a power of E.
-template <typename T, int NF, int C, int E, int SIGN, int L>
+template <typename T, int NF, int C, int E, int SIGN, int L, bool LM>
 __device__ __forceinline__ void fft_stages(C2<T> *v, int *pos, C2<T> *lds, const int t,
                                            const int c, const C2<T> *__restrict__ tw) {
   constexpr int REM = NF / L;  // remaining length to transform
@@ -222,19 +236,19 @@ __device__ __forceinline__ void fft_stages(C2<T> *v, int *pos, C2<T> *lds, const
     constexpr bool LAST = (REM == E);
     stage<T, NF, E, E, L, SIGN, LAST>(v, pos, t, tw);
     if constexpr (!LAST) {
-      exchange<T, NF, C, E>(v, pos, lds, t, c);
-      fft_stages<T, NF, C, E, SIGN, L * E>(v, pos, lds, t, c, tw);
+      exchange<T, NF, C, E, LM>(v, pos, lds, t, c);
+      fft_stages<T, NF, C, E, SIGN, L * E, LM>(v, pos, lds, t, c, tw);
     }
   } else if constexpr (REM > 1) {
     stage<T, NF, E, REM, L, SIGN, true>(v, pos, t, tw);
   }
 }
 
-template <typename T, int NF, int C, int E, int SIGN>
+template <typename T, int NF, int C, int E, int SIGN, bool LM = false>
 __device__ __forceinline__ void fft_regs(C2<T> *v, C2<T> *lds, const int t, const int c,
                                          const C2<T> *__restrict__ tw) {
   int pos[E];
-  fft_stages<T, NF, C, E, SIGN, 1>(v, pos, lds, t, c, tw);
+  fft_stages<T, NF, C, E, SIGN, 1, LM>(v, pos, lds, t, c, tw);
 }
 
 struct TileCtx {
@@ -290,12 +304,12 @@ __device__ __forceinline__ TileCtx tile_at(const pfk_pass &P, int C, int64_t ite
 // compiler wait for each load in turn.
 
 // Twiddle tables live in LDS when they fit next to the exchange buffer
-// (limit 144 KiB per workgroup): LDS reads count on lgkmcnt, so a twiddle
+// (limit 156 KiB per workgroup): LDS reads count on lgkmcnt, so a twiddle
 // read does not wait for the prefetched tile the way a global load would
 // (vmcnt completes in order).
 template <typename T>
-constexpr bool tw_in_lds(int nf, int c, int extra) {
-  return ((size_t)nf * c + nf + extra) * sizeof(C2<T>) <= 144 * 1024;
+constexpr bool tw_in_lds(int slots, int nf, int extra) {
+  return ((size_t)slots + nf + extra) * sizeof(C2<T>) <= 156 * 1024;
 }
 template <typename T>
 __device__ __forceinline__ void stage_table(C2<T> *dst, const C2<T> *src, int n) {
@@ -320,16 +334,19 @@ __device__ __forceinline__ void load_c2c(const pfk_pass &P, const T *src, const 
   }
 }
 
-template <typename T, int NF, int C, int E, int SIGN>
+// LM: line-major lane order (thread = c*T + t: a wave's lanes run along a
+// line), for passes whose lines are contiguous in memory; otherwise
+// thread = t*C + c (a wave's lanes run across the tile's lines).
+template <typename T, int NF, int C, int E, int SIGN, bool LM = false>
 __global__ void __launch_bounds__(C *NF / E)
 k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
            const T *__restrict__ src, T *__restrict__ dst) {
   constexpr int TT = NF / E;
-  constexpr bool TWL = tw_in_lds<T>(NF, C, 0);
-  __shared__ C2<T> lds[NF * C];
+  constexpr bool TWL = tw_in_lds<T>(lds_slots<NF, C, LM>(), NF, 0);
+  __shared__ C2<T> lds[lds_slots<NF, C, LM>()];
   __shared__ C2<T> twl[TWL ? NF : 1];
   const int tid = threadIdx.x;
-  const int c0 = tid % C, t0 = tid / C;
+  const int c0 = LM ? tid / TT : tid % C, t0 = LM ? tid % TT : tid / C;
   const C2<T> *twg = reinterpret_cast<const C2<T> *>(P.tw);
   if constexpr (TWL) stage_table(twl, twg, NF);
   const C2<T> *tw = TWL ? twl : twg;
@@ -356,7 +373,7 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       const T f = msign<T>(P.imod, j);
       cur[m] = ok ? C2<T>{cur[m].x * f, cur[m].y * f} : C2<T>{0, 0};
     }
-    fft_regs<T, NF, C, E, SIGN>(cur, lds, t, c, tw);
+    fft_regs<T, NF, C, E, SIGN, LM>(cur, lds, t, c, tw);
     if (valid) {
       C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
 #pragma unroll
@@ -403,7 +420,7 @@ k_r2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
            const T *__restrict__ src, T *__restrict__ dst) {
   constexpr int TT = NF / E;
   constexpr int H = NF;  // h = n/2
-  constexpr bool TWL = tw_in_lds<T>(NF, C, NF);
+  constexpr bool TWL = tw_in_lds<T>(NF * C, NF, NF);
   __shared__ C2<T> lds[NF * C];
   __shared__ C2<T> twl[TWL ? NF : 1], rtwl[TWL ? NF : 1];
   const int tid = threadIdx.x;
@@ -498,7 +515,7 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
            const T *__restrict__ src, T *__restrict__ dst) {
   constexpr int TT = NF / E;
   constexpr int H = NF;
-  constexpr bool TWL = tw_in_lds<T>(NF, C, C + NF);
+  constexpr bool TWL = tw_in_lds<T>((NF + 1) * C, NF, NF);
   __shared__ C2<T> lds[(NF + 1) * C];
   __shared__ C2<T> twl[TWL ? NF : 1], rtwl[TWL ? NF : 1];
   const int tid = threadIdx.x;
@@ -625,11 +642,18 @@ static int launch_items(void (*k)(pfk_pass, int64_t, int64_t, int64_t, const T *
 template <typename T, int NF, int C, int E, int SIGN>
 static int launch_kind(const pfk_pass *p, int64_t ntiles, const void *src, void *dst) {
   constexpr int NT = C * NF / E;
-  static int64_t grid_c2c = 0, grid_r2c = 0, grid_c2r = 0;   // per instantiation
+  static int64_t grid_c2c = 0, grid_lm = 0, grid_r2c = 0, grid_c2r = 0;   // per instantiation
   const bool persistent = !(p->variant & 4);
   const T *s = (const T *)src;
   T *d = (T *)dst;
   if (p->kind == PFK_C2C) {
+    if constexpr (sizeof(T) == 4) {   // line-major lane order: complex64 only
+      if (p->variant & 8) {
+        if (!grid_lm) grid_lm = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, true>, NT);
+        return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, true>, p, ntiles, grid_lm, persistent,
+                            NT, s, d);
+      }
+    }
     if (!grid_c2c) grid_c2c = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN>, NT);
     return launch_items(k_c2c_pow2<T, NF, C, E, SIGN>, p, ntiles, grid_c2c, persistent, NT, s, d);
   } else if (p->kind == PFK_R2C) {

@@ -70,7 +70,8 @@ typedef struct pfk_pass {
   int32_t mfma;       /* complex64 c2c arithmetic form: 0 plain, 1 FP32 MFMA, 2 split-BF16 MFMA */
   int32_t variant;    /* launch variant: 0 default, 1 general kernel, 2 narrow power-of-two
                          tiles, 3 short-length register kernel; + 4 (with 0 or 2): one
-                         workgroup per tile instead of the persistent grid */
+                         workgroup per tile instead of the persistent grid; + 8: line-major
+                         lane order (complex64 c2c) */
   int32_t pad2;
   /* device twiddle tables (owned by the host plan):
    *   tw  : W_nf^m = exp(sign*2*pi*i*m/nf), m in [0,nf)

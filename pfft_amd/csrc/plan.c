@@ -736,12 +736,18 @@ int pa_plan_npasses(const pa_plan *p) { return p ? p->npass : 0; }
 static const char *kernel_name(const pfk_pass *ps) {
   if (ps->kind == PFK_COPY) return "copy";
   const int fam = pfk_pass_kernels(ps);
-  if (fam & PFK_HAS_MFMA) return ps->mfma == 1 ? "mfma-f32" : "mfma-bf16x2";
+  const int lm = (ps->variant & 8) && ps->kind == PFK_C2C && ps->prec == PFK_F32;
+  if (fam & PFK_HAS_MFMA) {
+    static const char *nm[2][2] = {{"mfma-f32", "mfma-f32-lm"}, {"mfma-bf16x2", "mfma-bf16x2-lm"}};
+    return nm[ps->mfma == 2][lm && ps->nf <= 1024];
+  }
   if (ps->variant == 1) return "general";
   if (ps->variant == 3 && (fam & PFK_HAS_SMALL)) return "small";
   if (fam & PFK_HAS_POW2) {
-    static const char *nm[4] = {"fast", "fast-narrow", "fast-grid", "fast-narrow-grid"};
-    return nm[((ps->variant & 3) == 2 ? 1 : 0) + ((ps->variant & 4) ? 2 : 0)];
+    static const char *nm[2][4] = {
+        {"fast", "fast-narrow", "fast-grid", "fast-narrow-grid"},
+        {"fast-lm", "fast-narrow-lm", "fast-grid-lm", "fast-narrow-grid-lm"}};
+    return nm[lm][((ps->variant & 3) == 2 ? 1 : 0) + ((ps->variant & 4) ? 2 : 0)];
   }
   if (fam & PFK_HAS_SMALL) return "small";
   return "general";

@@ -17,13 +17,15 @@
  *   (2,0) (2,1) power-of-two register kernel, half the tile width;
  *   (4,r) (6,r) as (0,r) (2,r) with one workgroup per tile instead of
  *         the persistent grid;
+ *   (8,r) (12,r) as (0,r) (4,r) with line-major lane order (complex64 c2c
+ *         passes whose lines are contiguous in memory);
  *   (3,0) short-length kernel, one line per lane;
  *   (1,0) the general kernel.
  * A pass with one candidate is not timed.  All candidates compute the
  * same transform (the power-of-two variants bit-identically, the others
- * with their own factorisations).  A
- * pass run by an MFMA kernel keeps it: its arithmetic form is the user's
- * choice, not a tuning parameter.
+ * with their own factorisations).  A pass run by an MFMA kernel keeps it
+ * (its arithmetic form is the user's choice, not a tuning parameter); only
+ * its workgroup order is timed.
  *
  * Timing never touches the user's arrays: the plan passes the tuner
  * temporary device buffers of the same sizes.  Passes that move fewer
@@ -126,7 +128,7 @@ int pa_wisdom_import(const char *filename) {
     int v, r;
     double ms;
     if (!fgets(line, sizeof line, f) || sscanf(line, "%319s %d %d %lf", key, &v, &r, &ms) != 4 ||
-        v < 0 || v > 6 || v == 5 || r < 0 || r > 1) {
+        v < 0 || v > 14 || ((v & 1) && v != 1 && v != 3) || r < 0 || r > 1) {
       ok = 0;
       break;
     }
@@ -192,7 +194,7 @@ int pa_tune_lookup(pfk_pass *ps, unsigned effort) {
   const int mode = ev ? atoi(ev) : 1;   /* 0 off, 1 on, 2 on for every pass size */
   if (mode == 0 || (effort & PFFT_ESTIMATE) || ps->kind == PFK_COPY) return 0;
   const int fam = pfk_pass_kernels(ps);
-  if (fam == 0 || (fam & PFK_HAS_MFMA)) return 0;   /* general only, or the user's MFMA form */
+  if (fam == 0) return 0;   /* general kernel only */
   if (mode != 2 && pa_pass_bytes(ps) < PA_TUNE_MIN_BYTES) return 0;
   wisdom_env_load();
   char key[PA_KEY_LEN];
@@ -214,8 +216,10 @@ int pa_tune_measure(pfk_pass *ps, const void *src, void *dst, unsigned effort) {
   const char *rv = getenv("PFFT_AMD_XCD_REMAP");
   const int remap_fixed = (rv && *rv) ? (rv[0] == '1') : -1;
   const int reps = (effort & PFFT_EXHAUSTIVE) ? 10 : (effort & PFFT_PATIENT) ? 6 : 3;
-  static const int cand[10][2] = {{0, 0}, {0, 1}, {2, 0}, {2, 1}, {4, 0},
-                                  {4, 1}, {6, 0}, {6, 1}, {3, 0}, {1, 0}};
+  static const int cand[14][2] = {{0, 0}, {0, 1}, {2, 0},  {2, 1},  {4, 0},  {4, 1},  {6, 0},
+                                  {6, 1}, {8, 0}, {8, 1}, {12, 0}, {12, 1}, {3, 0}, {1, 0}};
+  /* line-major lane order: complex64 c2c passes whose lines are contiguous */
+  const int lm_ok = ps->kind == PFK_C2C && ps->prec == PFK_F32 && ps->sin == 1;
   const int fam = pfk_pass_kernels(ps);
   void *e0 = NULL, *e1 = NULL;
   if (pfk_event_create(&e0) || pfk_event_create(&e1)) {
@@ -225,10 +229,13 @@ int pa_tune_measure(pfk_pass *ps, const void *src, void *dst, unsigned effort) {
   const int v0 = ps->variant, r0 = ps->xcd_remap;
   int best_v = v0, best_r = r0, err = 0;
   double best = -1;
-  for (int c = 0; c < 10 && !err; c++) {
+  for (int c = 0; c < 14 && !err; c++) {
     const int v = cand[c][0], r = cand[c][1];
-    const int pow2 = v == 0 || v == 2 || v == 4 || v == 6;
-    if (pow2 && !(fam & PFK_HAS_POW2)) continue;
+    const int pow2 = v != 1 && v != 3;
+    if ((v & 8) && !lm_ok) continue;
+    if (fam & PFK_HAS_MFMA) {   /* the user's arithmetic form: grid, lane order, XCD order vary */
+      if ((v & 3) || (remap_fixed >= 0 && r != remap_fixed)) continue;
+    } else if (pow2 && !(fam & PFK_HAS_POW2)) continue;
     if (v == 3 && !(fam & PFK_HAS_SMALL)) continue;
     if (pow2 && remap_fixed >= 0 && r != remap_fixed) continue;
     ps->variant = v;

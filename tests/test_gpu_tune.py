@@ -75,12 +75,14 @@ def _expected(line, prec, variant, remap):
         return "small"
     if pow2:
         name = {0: "fast", 2: "fast-narrow", 3: "fast", 4: "fast-grid", 6: "fast-narrow-grid"}
-        return name[variant] + ("+xcd" if remap else "")
+        lm = (variant & 8) and kind == "c2c" and prec == "f32"
+        return name[variant & 7] + ("-lm" if lm else "") + ("+xcd" if remap else "")
     return "small" if small else "general"
 
 
 @pytest.mark.parametrize("variant,remap", [(0, 0), (0, 1), (2, 0), (2, 1), (4, 0), (4, 1),
-                                           (6, 0), (6, 1), (3, 0), (1, 0)])
+                                           (6, 0), (6, 1), (8, 0), (12, 1), (10, 0), (3, 0),
+                                           (1, 0)])
 @pytest.mark.parametrize("kind,n,flags,prec", CASES)
 def test_forced_variant_parity(tune_all, tmp_path, kind, n, flags, prec, variant, remap):
     spec, lines = _plan_lines(kind, n, flags, prec)     # tunes, fills the cache
@@ -164,3 +166,27 @@ def test_tuned_large_plan_roundtrip(monkeypatch):
     for b in (dx, dk, dy):
         b.free()
     P.forget_wisdom()
+
+
+@pytest.mark.parametrize("variant,remap", [(8, 0), (12, 1), (4, 0)])
+@pytest.mark.parametrize("form,tol,name", [(P.AMD_MFMA_F32, 5e-6, "mfma-f32"),
+                                           (P.AMD_MFMA_BF16X2, 1e-4, "mfma-bf16x2")])
+@pytest.mark.parametrize("n,flags", [([32, 256, 1024], TI), ([40, 512, 16], 0)])
+def test_forced_mfma_variants(tune_all, tmp_path, n, flags, form, tol, name, variant, remap):
+    """Every launch variant of the MFMA register kernel (line-major lane
+    order, per-tile grid, XCD order), forced through a wisdom file."""
+    spec, _ = _plan_lines("c2c", n, flags | form, "f32")
+    w = tmp_path / "w.txt"
+    assert P.export_wisdom(str(w)) == 0
+    _rewrite(str(w), variant, remap)
+    P.forget_wisdom()
+    assert P.import_wisdom(str(w)) == 0
+    _, lines = _plan_lines("c2c", n, flags | form, "f32")
+    for ln in lines:
+        if "kernel=mfma" in ln:
+            assert ln.endswith("kernel=" + name + ("-lm" if variant & 8 else "")), ln
+    for sign in (-1, +1):
+        spec = spec_of("c2c", n, flags=flags, sign=sign)
+        x = random_input(spec, 5).astype(np.float32).astype(np.float64)
+        y = run_gpu(spec, x, prec="f32", plan_flags_extra=form)
+        assert rel_l2(y, oracle_out(spec, x)) <= tol
