@@ -384,6 +384,8 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
   const int pad_rows = ev ? (ev[0] == '1') : 1;
   ev = getenv("PFFT_AMD_XCD_REMAP");
   const int xcd_remap = ev ? (ev[0] == '1') : 0;
+  ev = getenv("PFFT_AMD_INTER_ORDER");
+  const int inter_order = ev ? (ev[0] == '1') : 0;
   int done[PA_MAXRNK] = {0};
   arrgeo cur = p->gin, nxt;
   int64_t scratch_elems[2] = {0, 0};
@@ -403,6 +405,20 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
       nxt.rnk = R;
       nxt.h = howmany;
       order_identity(R, nxt.order);
+      /* Optional (PFFT_AMD_INTER_ORDER=1): place the next pass's line
+       * dimension ord[k+1] just outside the innermost one, so that pass
+       * reads its lines with a short stride (one padded row) and the long
+       * stride moves to this pass's stores.  Measured on MI355X: 5% faster
+       * for 1024x1024x512 complex128, slower for 512^3 r2c/c2r and for
+       * complex64, so it is off by default. */
+      const int dn = ord[k + 1];
+      if (inter_order && dn != last) {
+        int o = 0;
+        for (int t = 0; t < R; t++)
+          if (t != dn && t != last) nxt.order[o++] = t;
+        nxt.order[o++] = dn;
+        nxt.order[o++] = last;
+      }
       for (int t = 0; t < R; t++) nxt.ext[t] = done[t] ? ext_out_c[t] : ext_in_c[t];
       /* rows of the intermediate arrays start on 128-byte boundaries so the
        * 128-byte segments the pass kernels move are whole cache lines */
