@@ -200,6 +200,26 @@ typedef float pfftf_complex[2];
       const INT *iblock, const INT *oblock, MPI_Comm comm_cart, int pid,            \
       unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
       INT *local_o_start);                                                          \
+  /* r2r layout: api/pfft.h:163-166, 183-187, 207-212, 226-229, 246-250, 270-275 */ \
+  INT P##local_size_r2r_3d(const INT *n, MPI_Comm comm_cart, unsigned pfft_flags,   \
+      INT *local_ni, INT *local_i_start, INT *local_no, INT *local_o_start);        \
+  INT P##local_size_r2r(int rnk_n, const INT *n, MPI_Comm comm_cart,                \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  INT P##local_size_many_r2r(int rnk_n, const INT *n, const INT *ni,                \
+      const INT *no, INT howmany, const INT *iblock, const INT *oblock,             \
+      MPI_Comm comm_cart, unsigned pfft_flags, INT *local_ni,                       \
+      INT *local_i_start, INT *local_no, INT *local_o_start);                       \
+  void P##local_block_r2r_3d(const INT *n, MPI_Comm comm_cart, int pid,             \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  void P##local_block_r2r(int rnk_n, const INT *n, MPI_Comm comm_cart, int pid,     \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
+  void P##local_block_many_r2r(int rnk_n, const INT *ni, const INT *no,             \
+      const INT *iblock, const INT *oblock, MPI_Comm comm_cart, int pid,            \
+      unsigned pfft_flags, INT *local_ni, INT *local_i_start, INT *local_no,        \
+      INT *local_o_start);                                                          \
   /* planning: api/pfft.h:277-317 (basic/many), 319-343 (skipped) */                \
   P##plan P##plan_dft_3d(const INT *n, C *in, C *out, MPI_Comm comm_cart,           \
       int sign, unsigned pfft_flags);                                               \
@@ -234,6 +254,19 @@ typedef float pfftf_complex[2];
       const INT *no, INT howmany, const INT *iblock, const INT *oblock,             \
       const int *skip_trafos, C *in, R *out, MPI_Comm comm_cart, int sign,          \
       unsigned pfft_flags);                                                         \
+  /* r2r planning: api/pfft.h:286-288, 299-301, 318-322, 339-343.  Kinds:        \
+   * PFFT_DHT, PFFT_REDFT00..11, PFFT_RODFT00..11 (no SHIFTED flags) */             \
+  P##plan P##plan_r2r_3d(const INT *n, R *in, R *out, MPI_Comm comm_cart,           \
+      const P##r2r_kind *kinds, unsigned pfft_flags);                               \
+  P##plan P##plan_r2r(int rnk_n, const INT *n, R *in, R *out, MPI_Comm comm_cart,   \
+      const P##r2r_kind *kinds, unsigned pfft_flags);                               \
+  P##plan P##plan_many_r2r(int rnk_n, const INT *n, const INT *ni, const INT *no,   \
+      INT howmany, const INT *iblock, const INT *oblock, R *in, R *out,             \
+      MPI_Comm comm_cart, const P##r2r_kind *kinds, unsigned pfft_flags);           \
+  P##plan P##plan_many_r2r_skipped(int rnk_n, const INT *n, const INT *ni,          \
+      const INT *no, INT howmany, const INT *iblock, const INT *oblock,             \
+      const int *skip_trafos, R *in, R *out, MPI_Comm comm_cart,                    \
+      const P##r2r_kind *kinds, unsigned pfft_flags);                               \
   /* integer vector helpers: api/pfft.h:345-359 */                                  \
   INT  P##prod_INT(int d, const INT *vec);                                          \
   INT  P##sum_INT(int d, const INT *vec);                                           \

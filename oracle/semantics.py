@@ -18,6 +18,10 @@ Followed (reference file:line):
   serial DFTs                        FFTW 3.3.4 guru64 dft/r2c/c2r (kernel/sertrafo.c:489-554):
                                      unnormalised, exp(sign*2*pi*i*j*k/n); r2c forward only,
                                      c2r backward only, c2r ignores Im of DC/Nyquist
+  serial r2r                         FFTW guru64 r2r (kernel/sertrafo.c:531-537), one kind per
+                                     dimension (api/api-adv.c:141-155): REDFT00..11 and
+                                     RODFT00..11 are scipy's unnormalised DCT/DST types I-IV,
+                                     DHT = Re F - Im F of the forward DFT F
   memory layouts                     doc/tutorial.tex:304-330 (transposed decomposition)
 The DFTs use scipy.fft (pocketfft) in long double; pocketfft is pinned
 against brute-force DFT sums in tests/test_oracle_values.py.
@@ -39,9 +43,11 @@ class PlanSpec:
     """Parameters of a plan: what PX(plan_many_dft*) receives."""
 
     def __init__(self, kind, n, ni=None, no=None, howmany=1, sign=FORWARD, flags=0,
-                 rnk_pm=2, skip=None):
-        assert kind in ("c2c", "r2c", "c2r")
+                 rnk_pm=2, skip=None, kinds=None):
+        assert kind in ("c2c", "r2c", "c2r", "r2r")
+        assert (kind == "r2r") == (kinds is not None)
         self.kind = kind
+        self.kinds = list(kinds) if kinds is not None else None
         self.n = list(n)
         self.ni = list(ni if ni is not None else n)
         self.no = list(no if no is not None else n)
@@ -50,7 +56,7 @@ class PlanSpec:
         self.flags = flags
         self.rnk_pm = rnk_pm
         self.skip = list(skip) if skip is not None else [0] * len(self.n)
-        tf = {"c2c": L.T_C2C, "r2c": L.T_R2C, "c2r": L.T_C2R}[kind]
+        tf = {"c2c": L.T_C2C, "r2c": L.T_R2C, "c2r": L.T_C2R, "r2r": L.T_R2R}[kind]
         if kind != "c2c" and (flags & L.PADDED_R2C):
             tf |= L.T_PADDED
         self.tf = tf
@@ -76,10 +82,10 @@ class PlanSpec:
         return list(range(self.rnk))
 
     def in_is_real(self):
-        return self.kind == "r2c"
+        return self.kind in ("r2c", "r2r")
 
     def out_is_real(self):
-        return self.kind == "c2r"
+        return self.kind in ("c2r", "r2r")
 
     def in_logical_shape(self):
         s = list(self.ni)
@@ -298,6 +304,24 @@ def dft_c2r(arr, axis, n):
     return sfft.irfft(arr, n=n, axis=axis, workers=WORKERS) * n
 
 
+# FFTW r2r kind -> (scipy transform, type)
+R2R_KINDS = {3: ("dct", 1), 4: ("dct", 3), 5: ("dct", 2), 6: ("dct", 4),
+             7: ("dst", 1), 8: ("dst", 3), 9: ("dst", 2), 10: ("dst", 4)}
+DHT = 2
+
+
+def dft_r2r(arr, axis, kind):
+    """FFTW's unnormalised r2r of `kind` along `axis` (real in, real out)."""
+    a = np.real(arr)
+    if kind == DHT:
+        f = sfft.fft(a, axis=axis, workers=WORKERS)
+        return np.real(f) - np.imag(f)
+    fn, typ = R2R_KINDS[kind]
+    if fn == "dct":
+        return sfft.dct(a, type=typ, axis=axis, workers=WORKERS)
+    return sfft.dst(a, type=typ, axis=axis, workers=WORKERS)
+
+
 # ---------------------------------------------------------------- execute
 def execute(spec, x_in, precision=np.longdouble):
     """Run the plan on the user input `x_in` (flat array in the input's
@@ -355,7 +379,9 @@ def execute(spec, x_in, precision=np.longdouble):
             # c2r last (FFTW rdft2 semantics)
             for d in (reversed(dims) if (tf & L.T_R2C) else dims):
                 last = d == R - 1
-                if (tf & L.T_R2C) and last:
+                if tf & L.T_R2R:
+                    a = dft_r2r(a, d, spec.kinds[d])
+                elif (tf & L.T_R2C) and last:
                     a = dft_r2c(a, d, n_[d])
                 elif (tf & L.T_C2R) and last:
                     a = dft_c2r(a, d, n_[d])

@@ -25,6 +25,9 @@ SHIFTED_IN = 1 << 2
 SHIFTED_OUT = 1 << 3
 MEASURE = 0
 ESTIMATE = 1 << 4
+# r2r kinds (FFTW numbering)
+DHT, REDFT00, REDFT01, REDFT10, REDFT11 = 2, 3, 4, 5, 6
+RODFT00, RODFT01, RODFT10, RODFT11 = 7, 8, 9, 10
 PATIENT = 1 << 5
 EXHAUSTIVE = 1 << 6
 TUNE = 1 << 7
@@ -64,7 +67,8 @@ def lib():
 
 def _declare(L):
     for P in ("pfft_", "pfftf_"):
-        for nm in ("local_size_many_dft", "local_size_many_dft_r2c", "local_size_many_dft_c2r"):
+        for nm in ("local_size_many_dft", "local_size_many_dft_r2c", "local_size_many_dft_c2r",
+                   "local_size_many_r2r"):
             f = getattr(L, P + nm)
             f.restype = INT
             f.argtypes = [ctypes.c_int, INTP, INTP, INTP, INT, INTP, INTP, ctypes.c_int,
@@ -74,7 +78,7 @@ def _declare(L):
             f.restype = INT
             f.argtypes = [INTP, ctypes.c_int, ctypes.c_uint, INTP, INTP, INTP, INTP]
         for nm in ("local_block_many_dft", "local_block_many_dft_r2c",
-                   "local_block_many_dft_c2r"):
+                   "local_block_many_dft_c2r", "local_block_many_r2r"):
             f = getattr(L, P + nm)
             f.restype = None
             f.argtypes = [ctypes.c_int, INTP, INTP, INTP, INTP, ctypes.c_int, ctypes.c_int,
@@ -91,10 +95,20 @@ def _declare(L):
             f.argtypes = [ctypes.c_int, INTP, INTP, INTP, INT, INTP, INTP,
                           ctypes.POINTER(ctypes.c_int), VP, VP, ctypes.c_int, ctypes.c_int,
                           ctypes.c_uint]
+        f = getattr(L, P + "plan_many_r2r")
+        f.restype = VP
+        f.argtypes = [ctypes.c_int, INTP, INTP, INTP, INT, INTP, INTP, VP, VP, ctypes.c_int,
+                      ctypes.POINTER(ctypes.c_int), ctypes.c_uint]
+        f = getattr(L, P + "plan_many_r2r_skipped")
+        f.restype = VP
+        f.argtypes = [ctypes.c_int, INTP, INTP, INTP, INT, INTP, INTP,
+                      ctypes.POINTER(ctypes.c_int), VP, VP, ctypes.c_int,
+                      ctypes.POINTER(ctypes.c_int), ctypes.c_uint]
         for nm in ("execute",):
             getattr(L, P + nm).argtypes = [VP]
             getattr(L, P + nm).restype = None
-        for nm in ("execute_dft", "execute_dft_r2c", "execute_dft_c2r", "execute_adjoint"):
+        for nm in ("execute_dft", "execute_dft_r2c", "execute_dft_c2r", "execute_r2r",
+                   "execute_adjoint"):
             getattr(L, P + nm).argtypes = [VP, VP, VP]
             getattr(L, P + nm).restype = None
         getattr(L, P + "execute_batch").argtypes = [ctypes.POINTER(VP), ctypes.c_int]
@@ -169,14 +183,15 @@ def create_procmesh(np_dims, comm=COMM_WORLD):
 # ---------------------------------------------------------------- layout
 def local_size(kind, n, ni=None, no=None, howmany=1, iblock=None, oblock=None,
                comm=COMM_WORLD, flags=0, prec="f64"):
-    """pfft_local_size_many_dft{,_r2c,_c2r}: returns
+    """pfft_local_size_many_dft{,_r2c,_c2r} / pfft_local_size_many_r2r: returns
     (alloc_local, local_ni, local_i_start, local_no, local_o_start)."""
     rnk = len(n)
     ni = n if ni is None else ni
     no = n if no is None else no
     lni, lis, lno, los = (INT * rnk)(), (INT * rnk)(), (INT * rnk)(), (INT * rnk)()
-    suffix = {"c2c": "", "r2c": "_r2c", "c2r": "_c2r"}[kind]
-    f = getattr(lib(), _prefix(prec) + "local_size_many_dft" + suffix)
+    name = {"c2c": "local_size_many_dft", "r2c": "local_size_many_dft_r2c",
+            "c2r": "local_size_many_dft_c2r", "r2r": "local_size_many_r2r"}[kind]
+    f = getattr(lib(), _prefix(prec) + name)
     mem = f(rnk, ivec(n), ivec(ni), ivec(no), howmany, ivec(iblock), ivec(oblock), comm, flags,
             lni, lis, lno, los)
     return int(mem), list(lni), list(lis), list(lno), list(los)
@@ -187,8 +202,9 @@ def local_block(kind, ni, no=None, iblock=None, oblock=None, comm=COMM_WORLD, pi
     rnk = len(ni)
     no = ni if no is None else no
     lni, lis, lno, los = (INT * rnk)(), (INT * rnk)(), (INT * rnk)(), (INT * rnk)()
-    suffix = {"c2c": "", "r2c": "_r2c", "c2r": "_c2r"}[kind]
-    f = getattr(lib(), _prefix(prec) + "local_block_many_dft" + suffix)
+    name = {"c2c": "local_block_many_dft", "r2c": "local_block_many_dft_r2c",
+            "c2r": "local_block_many_dft_c2r", "r2r": "local_block_many_r2r"}[kind]
+    f = getattr(lib(), _prefix(prec) + name)
     f(rnk, ivec(ni), ivec(no), ivec(iblock), ivec(oblock), comm, pid, flags, lni, lis, lno, los)
     return list(lni), list(lis), list(lno), list(los)
 
@@ -265,15 +281,28 @@ class Plan:
     """A pfft plan handle (pfft_plan_many_dft* / pfftf_*)."""
 
     def __init__(self, kind, n, in_ptr, out_ptr, ni=None, no=None, howmany=1, sign=FORWARD,
-                 flags=0, comm=COMM_WORLD, iblock=None, oblock=None, skip=None, prec="f64"):
+                 flags=0, comm=COMM_WORLD, iblock=None, oblock=None, skip=None, prec="f64",
+                 kinds=None):
         self.kind = kind
         self.prec = prec
         self.P = _prefix(prec)
         rnk = len(n)
         ni = n if ni is None else ni
         no = n if no is None else no
-        suffix = {"c2c": "", "r2c": "_r2c", "c2r": "_c2r"}[kind]
         L = lib()
+        if kind == "r2r":
+            kv = (ctypes.c_int * rnk)(*kinds)
+            if skip is None:
+                self.handle = getattr(L, self.P + "plan_many_r2r")(
+                    rnk, ivec(n), ivec(ni), ivec(no), howmany, ivec(iblock), ivec(oblock),
+                    in_ptr, out_ptr, comm, kv, flags)
+            else:
+                sk = (ctypes.c_int * rnk)(*skip)
+                self.handle = getattr(L, self.P + "plan_many_r2r_skipped")(
+                    rnk, ivec(n), ivec(ni), ivec(no), howmany, ivec(iblock), ivec(oblock), sk,
+                    in_ptr, out_ptr, comm, kv, flags)
+            return
+        suffix = {"c2c": "", "r2c": "_r2c", "c2r": "_c2r"}[kind]
         if skip is None:
             f = getattr(L, self.P + "plan_many_dft" + suffix)
             self.handle = f(rnk, ivec(n), ivec(ni), ivec(no), howmany, ivec(iblock), ivec(oblock),
@@ -298,7 +327,7 @@ class Plan:
         return buf.value.decode().strip().split("\n")
 
     def execute_on(self, in_ptr, out_ptr):
-        suffix = {"c2c": "dft", "r2c": "dft_r2c", "c2r": "dft_c2r"}[self.kind]
+        suffix = {"c2c": "dft", "r2c": "dft_r2c", "c2r": "dft_c2r", "r2r": "r2r"}[self.kind]
         getattr(lib(), self.P + "execute_" + suffix)(self.handle, in_ptr, out_ptr)
 
     def execute_adjoint(self, in_ptr, out_ptr):

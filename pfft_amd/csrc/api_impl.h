@@ -27,7 +27,37 @@ static PA(plan) PA(mk)(int rnk_n, const INT *n, const INT *ni, const INT *no, IN
                        MPI_Comm comm, int sign, const int *skip, unsigned tf,
                        unsigned pfft_flags) {
   return (PA(plan))pa_plan_create(PA_PREC, rnk_n, n, ni, no, howmany, iblock, oblock, in, out,
-                                  comm, sign, skip, tf, pfft_flags);
+                                  comm, sign, skip, tf, pfft_flags, NULL);
+}
+
+/* r2r plans: api/api-adv.c:141-155, 204-217 (sign 0, one kind per dim) */
+PA(plan) PA(plan_many_r2r_skipped)(int rnk_n, const INT *n, const INT *ni, const INT *no,
+                                   INT howmany, const INT *iblock, const INT *oblock,
+                                   const int *skip_trafos, R *in, R *out, MPI_Comm comm_cart,
+                                   const PA(r2r_kind) *kinds, unsigned pfft_flags) {
+  int k[PA_MAXRNK];
+  if (rnk_n < 1 || rnk_n > PA_MAXRNK || !kinds) {
+    pa_error("r2r plans need 1..%d dimensions and one kind per dimension", PA_MAXRNK);
+    return NULL;
+  }
+  for (int t = 0; t < rnk_n; t++) k[t] = (int)kinds[t];
+  return (PA(plan))pa_plan_create(PA_PREC, rnk_n, n, ni, no, howmany, iblock, oblock, in, out,
+                                  comm_cart, 0, skip_trafos, TF_R2R, pfft_flags, k);
+}
+PA(plan) PA(plan_many_r2r)(int rnk_n, const INT *n, const INT *ni, const INT *no, INT howmany,
+                           const INT *iblock, const INT *oblock, R *in, R *out,
+                           MPI_Comm comm_cart, const PA(r2r_kind) *kinds, unsigned pfft_flags) {
+  return PA(plan_many_r2r_skipped)(rnk_n, n, ni, no, howmany, iblock, oblock, NULL, in, out,
+                                   comm_cart, kinds, pfft_flags);
+}
+PA(plan) PA(plan_r2r)(int rnk_n, const INT *n, R *in, R *out, MPI_Comm comm_cart,
+                      const PA(r2r_kind) *kinds, unsigned pfft_flags) {
+  return PA(plan_many_r2r)(rnk_n, n, n, n, 1, PFFT_DEFAULT_BLOCKS, PFFT_DEFAULT_BLOCKS, in, out,
+                           comm_cart, kinds, pfft_flags);
+}
+PA(plan) PA(plan_r2r_3d)(const INT *n, R *in, R *out, MPI_Comm comm_cart,
+                         const PA(r2r_kind) *kinds, unsigned pfft_flags) {
+  return PA(plan_r2r)(3, n, in, out, comm_cart, kinds, pfft_flags);
 }
 
 PA(plan) PA(plan_many_dft)(int rnk_n, const INT *n, const INT *ni, const INT *no, INT howmany,
@@ -116,9 +146,8 @@ void PA(execute_dft_c2r)(const PA(plan) plan, CPLX *in, R *out) {
   pa_plan_execute((const pa_plan *)plan, in, out);
 }
 void PA(execute_r2r)(const PA(plan) plan, R *in, R *out) {
-  (void)in; (void)out;
   if (!plan) { fprintf(stderr, "!!! Error: Can not execute PFFT Plan == NULL !!!\n"); return; }
-  pa_error("r2r transforms are not supported");
+  pa_plan_execute((const pa_plan *)plan, in, out);
 }
 void PA(execute_adjoint)(const PA(plan) plan, void *in, void *out) {
   if (!plan) { fprintf(stderr, "!!! Error: Can not execute PFFT Plan == NULL !!!\n"); return; }
@@ -194,6 +223,23 @@ INT PA(local_size_dft_c2r_3d)(const INT *n, MPI_Comm comm_cart, unsigned pfft_fl
   return PA(local_size_dft_c2r)(3, n, comm_cart, pfft_flags, lni, lis, lno, los);
 }
 
+/* r2r layout: api/api-adv.c:76-88, api/api-basic.c:775-785, 929-940 */
+INT PA(local_size_many_r2r)(int rnk_n, const INT *n, const INT *ni, const INT *no, INT howmany,
+                            const INT *iblock, const INT *oblock, MPI_Comm comm_cart,
+                            unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  return PA(ls)(rnk_n, n, ni, no, howmany, iblock, oblock, comm_cart, TF_R2R, pfft_flags, lni,
+                lis, lno, los);
+}
+INT PA(local_size_r2r)(int rnk_n, const INT *n, MPI_Comm comm_cart, unsigned pfft_flags, INT *lni,
+                       INT *lis, INT *lno, INT *los) {
+  return PA(local_size_many_r2r)(rnk_n, n, n, n, 1, PFFT_DEFAULT_BLOCKS, PFFT_DEFAULT_BLOCKS,
+                                 comm_cart, pfft_flags, lni, lis, lno, los);
+}
+INT PA(local_size_r2r_3d)(const INT *n, MPI_Comm comm_cart, unsigned pfft_flags, INT *lni,
+                          INT *lis, INT *lno, INT *los) {
+  return PA(local_size_r2r)(3, n, comm_cart, pfft_flags, lni, lis, lno, los);
+}
+
 static void PA(lb)(int rnk_n, const INT *ni, const INT *no, const INT *iblock, const INT *oblock,
                    MPI_Comm comm, int pid, unsigned tf, unsigned pfft_flags, INT *lni, INT *lis,
                    INT *lno, INT *los) {
@@ -217,6 +263,21 @@ void PA(local_block_many_dft_c2r)(int rnk_n, const INT *ni, const INT *no, const
                                   unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
   PA(lb)(rnk_n, ni, no, iblock, oblock, comm_cart, pid, c2r_flag(pfft_flags), pfft_flags, lni,
          lis, lno, los);
+}
+/* r2r blocks: api/api-adv.c:373-385, api/api-basic.c:872-882, 1029-1037 */
+void PA(local_block_many_r2r)(int rnk_n, const INT *ni, const INT *no, const INT *iblock,
+                              const INT *oblock, MPI_Comm comm_cart, int pid,
+                              unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(lb)(rnk_n, ni, no, iblock, oblock, comm_cart, pid, TF_R2R, pfft_flags, lni, lis, lno, los);
+}
+void PA(local_block_r2r)(int rnk_n, const INT *n, MPI_Comm comm_cart, int pid,
+                         unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(local_block_many_r2r)(rnk_n, n, n, PFFT_DEFAULT_BLOCKS, PFFT_DEFAULT_BLOCKS, comm_cart, pid,
+                           pfft_flags, lni, lis, lno, los);
+}
+void PA(local_block_r2r_3d)(const INT *n, MPI_Comm comm_cart, int pid, unsigned pfft_flags,
+                            INT *lni, INT *lis, INT *lno, INT *los) {
+  PA(local_block_r2r)(3, n, comm_cart, pid, pfft_flags, lni, lis, lno, los);
 }
 void PA(local_block_dft)(int rnk_n, const INT *n, MPI_Comm comm_cart, int pid,
                          unsigned pfft_flags, INT *lni, INT *lis, INT *lno, INT *los) {

@@ -55,7 +55,7 @@ typedef struct pa_plan pa_plan;
 pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const INT *no,
                         INT howmany, const INT *iblock, const INT *oblock, void *in,
                         void *out, MPI_Comm comm, int sign, const int *skip_trafos,
-                        unsigned trafo_flag, unsigned pfft_flags);
+                        unsigned trafo_flag, unsigned pfft_flags, const int *kinds);
 void pa_plan_execute(const pa_plan *p, void *in, void *out);
 void pa_plan_execute_adjoint(const pa_plan *p, void *in, void *out);
 void pa_plan_destroy(pa_plan *p);

@@ -46,6 +46,7 @@ k_batch_generic(const pfk_task *__restrict__ tasks, const int t0, const int t1,
       case PFK_C2C: pass_tile<T, PFK_C2C, 16>(tk.pass, tk.C, tk.pitch, tile, src, dst, smem_raw); break;
       case PFK_R2C: pass_tile<T, PFK_R2C, 16>(tk.pass, tk.C, tk.pitch, tile, src, dst, smem_raw); break;
       case PFK_C2R: pass_tile<T, PFK_C2R, 16>(tk.pass, tk.C, tk.pitch, tile, src, dst, smem_raw); break;
+      case PFK_R2R: pass_tile<T, PFK_R2R, 16>(tk.pass, tk.C, tk.pitch, tile, src, dst, smem_raw); break;
       default: pass_tile<T, PFK_COPY, 16>(tk.pass, tk.C, tk.pitch, tile, src, dst, smem_raw); break;
     }
     lds_barrier();   // LDS is reused by the next item

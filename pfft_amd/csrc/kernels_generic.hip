@@ -44,6 +44,7 @@ k_pass_generic(const pfk_pass P, const int C, const int pitch, const int64_t til
 
 // number of complex LDS slots one line needs
 static int64_t slots_per_line(const pfk_pass *p) {
+  if (p->kind == PFK_R2R) return p->nf;   // the extended line
   if (p->kind == PFK_R2C) return (p->n % 2 == 0) ? p->n / 2 : p->n;
   if (p->kind == PFK_C2R) return (p->n % 2 == 0) ? p->n / 2 + 1 : p->n;
   return p->n;
@@ -95,6 +96,10 @@ static int launch_generic_t(const pfk_pass *p, const void *src, void *dst) {
         break;
       case PFK_C2R:
         hipLaunchKernelGGL((k_pass_generic<T, PFK_C2R, EPT>), grid, block, lds, g_stream, *p, C,
+                           (int)pitch, base, s, d);
+        break;
+      case PFK_R2R:
+        hipLaunchKernelGGL((k_pass_generic<T, PFK_R2R, EPT>), grid, block, lds, g_stream, *p, C,
                            (int)pitch, base, s, d);
         break;
       default:

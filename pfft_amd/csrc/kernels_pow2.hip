@@ -733,7 +733,7 @@ static int launch_pow2_f32_alt(const pfk_pass *p, const void *src, void *dst) {
 }
 
 int fast_covers(const pfk_pass *p) {
-  if (p->kind == PFK_COPY) return 0;
+  if (p->kind != PFK_C2C && p->kind != PFK_R2C && p->kind != PFK_C2R) return 0;
   const int64_t nf = p->nf;
   if (nf < 64 || nf > 4096 || (nf & (nf - 1))) return 0;
   if (p->kind == PFK_R2C) {

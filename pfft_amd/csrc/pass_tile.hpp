@@ -87,6 +87,44 @@ __device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const 
           const T f = (T)mod_sign(P.imod, j);
           buf[cc * pitch + zin + j] = C2{v.x * f, v.y * f};
         });
+  } else if (KIND == PFK_R2R) {
+    // real line x (length n, embedded at zin) -> extended real sequence u of
+    // length nf in complex slots (layout in pfk.h)
+    const int kd = P.r2r;
+    for (int idx = tid; idx < nl * nf; idx += NT) {
+      const int cc = idx / nf, p = idx - cc * nf;
+      buf[cc * pitch + p] = C2{0, 0};
+    }
+    lds_barrier();
+    batched<8, T>(
+        tid, NT, nl * nin,
+        [&](int idx) {
+          const int cc = ld_tile_fast ? idx % nl : idx / nin;
+          const int j = ld_tile_fast ? idx / nl : idx % nin;
+          return src[sb + cc * P.tsin + (int64_t)j * P.sin];
+        },
+        [&](int idx, T v) {
+          const int cc = ld_tile_fast ? idx % nl : idx / nin;
+          const int j = ld_tile_fast ? idx / nl : idx % nin;
+          const T x = v * (T)mod_sign(P.imod, j);
+          const int p = zin + j;
+          C2 *L = buf + cc * pitch;
+          int a = -1, b = -1;
+          T sb_ = (T)1;   // sign of the mirrored entry
+          switch (kd) {
+            case PFK_DHT: a = p; break;
+            case PFK_REDFT00: a = p; if (p > 0 && p < n - 1) b = nf - p; break;
+            case PFK_REDFT10: a = 2 * p + 1; b = nf - 2 * p - 1; break;
+            case PFK_REDFT01: a = p; if (p > 0) b = nf - p; break;
+            case PFK_REDFT11: a = 2 * p + 1; b = nf - 2 * p - 1; break;
+            case PFK_RODFT00: a = p + 1; b = nf - p - 1; sb_ = (T)-1; break;
+            case PFK_RODFT10: a = 2 * p + 1; b = nf - 2 * p - 1; sb_ = (T)-1; break;
+            case PFK_RODFT01: a = p + 1; if (p < n - 1) b = nf - p - 1; sb_ = (T)-1; break;
+            default: a = 2 * p + 1; b = nf - 2 * p - 1; sb_ = (T)-1; break;   // RODFT11
+          }
+          if (a >= 0) L[a] = C2{x, (T)0};
+          if (b >= 0) L[b] = C2{sb_ * x, (T)0};
+        });
   } else if (KIND == PFK_R2C) {
     if ((n & 1) == 0) {
       // real slot p of line cc is rbuf[2*cc*pitch + p]: complex slot p/2
@@ -261,6 +299,24 @@ __device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const 
       const C2 v = buf[cc * pitch + zout + k];
       const T f = (T)mod_sign(P.omod, k);
       d[db + cc * P.tsout + (int64_t)k * P.sout] = C2{v.x * f, v.y * f};
+    }
+  } else if (KIND == PFK_R2R) {
+    const int kd = P.r2r;
+    for (int idx = tid; idx < nl * nout; idx += NT) {
+      int cc, k;
+      if (st_tile_fast) { cc = idx % nl; k = idx / nl; }
+      else { k = idx % nout; cc = idx / nout; }
+      const int kk = zout + k;
+      const C2 *L = buf + cc * pitch;
+      T y;
+      switch (kd) {
+        case PFK_DHT: y = L[kk].x - L[kk].y; break;
+        case PFK_REDFT00: case PFK_REDFT10: y = L[kk].x; break;
+        case PFK_REDFT01: case PFK_REDFT11: y = L[2 * kk + 1].x; break;
+        case PFK_RODFT00: case PFK_RODFT10: y = -L[kk + 1].y; break;
+        default: y = -L[2 * kk + 1].y; break;   // RODFT01, RODFT11
+      }
+      dst[db + cc * P.tsout + (int64_t)k * P.sout] = y * (T)mod_sign(P.omod, k);
     }
   } else if (KIND == PFK_R2C) {
     C2 *d = reinterpret_cast<C2 *>(dst);

@@ -37,7 +37,26 @@ extern "C" {
 #define PFK_MAXB 6
 #define PFK_MAXSTAGES 24
 
-enum pfk_kind { PFK_C2C = 0, PFK_R2C = 1, PFK_C2R = 2, PFK_COPY = 3 };
+enum pfk_kind { PFK_C2C = 0, PFK_R2C = 1, PFK_C2R = 2, PFK_COPY = 3, PFK_R2R = 4 };
+
+/* Real-to-real kinds of a PFK_R2R pass (the FFTW kind numbers of
+ * include/pfft.h).  A length-n r2r line is computed as a forward complex
+ * DFT of length nf of a real sequence u built from the line:
+ *   DHT      nf = n      u_p = x_p                         Y_k = Re F_k - Im F_k
+ *   REDFT00  nf = 2(n-1) u_p = u_{nf-p} = x_p              Y_k = Re F_k
+ *   REDFT10  nf = 4n     u_{2p+1} = u_{nf-2p-1} = x_p      Y_k = Re F_k
+ *   REDFT01  nf = 4n     u_p = u_{nf-p} = x_p              Y_k = Re F_{2k+1}
+ *   REDFT11  nf = 8n     u_{2p+1} = u_{nf-2p-1} = x_p      Y_k = Re F_{2k+1}
+ *   RODFT00  nf = 2(n+1) u_{p+1} = -u_{nf-p-1} = x_p       Y_k = -Im F_{k+1}
+ *   RODFT10  nf = 4n     u_{2p+1} = -u_{nf-2p-1} = x_p     Y_k = -Im F_{k+1}
+ *   RODFT01  nf = 4n     u_{p+1} = -u_{nf-p-1} = x_p (p < n-1),
+ *                        u_n = x_{n-1} (unmirrored)        Y_k = -Im F_{2k+1}
+ *   RODFT11  nf = 8n     u_{2p+1} = -u_{nf-2p-1} = x_p     Y_k = -Im F_{2k+1}
+ * (all other u entries zero).  Each identity is the kind's defining sum
+ * with the cosine/sine written as the sum of two exponentials. */
+enum pfk_r2r { PFK_DHT = 2, PFK_REDFT00 = 3, PFK_REDFT01 = 4, PFK_REDFT10 = 5,
+               PFK_REDFT11 = 6, PFK_RODFT00 = 7, PFK_RODFT01 = 8, PFK_RODFT10 = 9,
+               PFK_RODFT11 = 10 };
 enum pfk_prec { PFK_F64 = 0, PFK_F32 = 1 };
 
 /* +-1 modulation of a line index:
@@ -56,7 +75,8 @@ typedef struct pfk_pass {
   int32_t nstages;    /* radix factorisation of the computed FFT length */
   int32_t radix[PFK_MAXSTAGES];
   int64_t n;          /* logical transform length */
-  int64_t nf;         /* computed complex FFT length (n, or n/2 for even r2c/c2r) */
+  int64_t nf;         /* computed complex FFT length (n, n/2 for even r2c/c2r, the
+                         extended length for r2r) */
   int64_t nin, zin;   /* input count along the line and embed offset */
   int64_t nout, zout; /* kept output count and first kept index */
   int64_t sin, sout;  /* line strides, in elements of the src/dst type */
@@ -72,7 +92,7 @@ typedef struct pfk_pass {
                          tiles, 3 short-length register kernel; + 4 (with 0 or 2): one
                          workgroup per tile instead of the persistent grid; + 8: line-major
                          lane order (complex64 c2c) */
-  int32_t pad2;
+  int32_t r2r;        /* enum pfk_r2r of a PFK_R2R pass */
   /* device twiddle tables (owned by the host plan):
    *   tw  : W_nf^m = exp(sign*2*pi*i*m/nf), m in [0,nf)
    *   rtw : W_n^k  = exp(sign*2*pi*i*k/n),  k in [0,n/2] (even r2c/c2r only) */

@@ -61,6 +61,7 @@ struct pa_plan {
   void *scratch[2];
   size_t scratch_bytes[2];
   const void *tw_adj[PA_MAXRNK], *rtw_adj[PA_MAXRNK];   /* opposite-sign tables */
+  int is_r2r;
   void *tw_bufs[4 * PA_MAXRNK];
   int ntw;
 };
@@ -254,10 +255,23 @@ static void tune_plan(pa_plan *p) {
   pfk_free(tout);
 }
 
+/* Extended complex FFT length of an r2r line of length n (pfk.h), 0 for
+ * an unsupported kind. */
+static INT r2r_length(int kind, INT n) {
+  switch (kind) {
+    case PFK_DHT: return n;
+    case PFK_REDFT00: return 2 * (n - 1);
+    case PFK_RODFT00: return 2 * (n + 1);
+    case PFK_REDFT10: case PFK_REDFT01: case PFK_RODFT10: case PFK_RODFT01: return 4 * n;
+    case PFK_REDFT11: case PFK_RODFT11: return 8 * n;
+    default: return 0;
+  }
+}
+
 pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const INT *no,
                         INT howmany, const INT *iblock, const INT *oblock, void *in, void *out,
                         MPI_Comm comm, int sign, const int *skip_trafos, unsigned tf,
-                        unsigned pfft_flags) {
+                        unsigned pfft_flags, const int *kinds) {
   const unsigned transp = pfft_flags & (PFFT_TRANSPOSED_IN | PFFT_TRANSPOSED_OUT);
   pa_mesh mesh;
 
@@ -279,7 +293,20 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     pa_error("3-d process meshes (3d-to-2d remap) are not supported on one GPU");
     return NULL;
   }
-  if (tf & TF_R2R) { pa_error("r2r transforms are not supported"); return NULL; }
+  if (tf & TF_R2R) {
+    if (!kinds) { pa_error("r2r plans need one kind per dimension"); return NULL; }
+    for (int t = 0; t < rnk_n; t++) {
+      const INT m = r2r_length(kinds[t], n[t]);
+      if (m <= 0) {
+        pa_error("r2r kind %d of dimension %d is not supported (DHT, REDFT, RODFT)", kinds[t], t);
+        return NULL;
+      }
+      if (m > 4096) {   /* the general kernel keeps a line in one workgroup */
+        pa_error("r2r dimension %d: extended length %lld exceeds one workgroup", t, (long long)m);
+        return NULL;
+      }
+    }
+  }
   if ((pfft_flags & PFFT_AMD_MFMA_F32) && (pfft_flags & PFFT_AMD_MFMA_BF16X2)) {
     pa_error("choose one of PFFT_AMD_MFMA_F32 and PFFT_AMD_MFMA_BF16X2");
     return NULL;
@@ -342,9 +369,11 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
 
   const int R = rnk_n, last = R - 1;
   const int is_r2c = (tf & TF_R2C) != 0, is_c2r = (tf & TF_C2R) != 0;
+  const int is_r2r = (tf & TF_R2R) != 0;
   const int padded = (tf & TF_PADDED) != 0;
-  p->in_esz = (prec == PFK_F64 ? 8 : 4) * (is_r2c ? 1 : 2);
-  p->out_esz = (prec == PFK_F64 ? 8 : 4) * (is_c2r ? 1 : 2);
+  p->is_r2r = is_r2r;
+  p->in_esz = (prec == PFK_F64 ? 8 : 4) * ((is_r2c || is_r2r) ? 1 : 2);
+  p->out_esz = (prec == PFK_F64 ? 8 : 4) * ((is_c2r || is_r2r) ? 1 : 2);
 
   /* user arrays */
   p->gin.rnk = p->gout.rnk = R;
@@ -372,6 +401,7 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     p->gin.ext[last] = ni[last] / 2 + 1;
     ext_in_c[last] = ni[last] / 2 + 1;
   }
+  if (is_r2r) p->gin.is_real = p->gout.is_real = 1;
   p->in_elems = geo_elems(&p->gin);
   p->out_elems = geo_elems(&p->gout);
 
@@ -420,9 +450,10 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
         nxt.order[o++] = last;
       }
       for (int t = 0; t < R; t++) nxt.ext[t] = done[t] ? ext_out_c[t] : ext_in_c[t];
+      nxt.is_real = is_r2r;   /* r2r intermediates are real */
       /* rows of the intermediate arrays start on 128-byte boundaries so the
        * 128-byte segments the pass kernels move are whole cache lines */
-      const INT align = (prec == PFK_F64) ? 8 : 16;
+      const INT align = (prec == PFK_F64 ? 8 : 16) * (is_r2r ? 2 : 1);
       if (pad_rows && howmany == 1 && nxt.ext[last] >= align && nxt.ext[last] % align)
         nxt.phys_last = (nxt.ext[last] + align - 1) / align * align;
     }
@@ -431,13 +462,18 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     if (d == last && is_r2c) kind = PFK_R2C;
     else if (d == last && is_c2r) kind = PFK_C2R;
     else if (skipped) kind = PFK_COPY;
+    else if (is_r2r) kind = PFK_R2R;
     ps->kind = kind;
     ps->xcd_remap = xcd_remap;
     ps->mfma = (pfft_flags & PFFT_AMD_MFMA_F32) ? 1 : (pfft_flags & PFFT_AMD_MFMA_BF16X2) ? 2 : 0;
     ps->prec = prec;
-    ps->sign = sign;
+    ps->sign = (kind == PFK_R2R) ? PFFT_FORWARD : sign;
     ps->n = n[d];
     ps->nf = ((kind == PFK_R2C || kind == PFK_C2R) && n[d] % 2 == 0) ? n[d] / 2 : n[d];
+    if (kind == PFK_R2R) {
+      ps->r2r = kinds[d];
+      ps->nf = r2r_length(kinds[d], n[d]);
+    }
     /* embed / trunc offsets: kernel/ousample.c:241-322 */
     const int sh_in = (pfft_flags & PFFT_SHIFTED_IN) != 0;
     const int sh_out = (pfft_flags & PFFT_SHIFTED_OUT) != 0;
@@ -465,8 +501,8 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     if (kind != PFK_COPY) {
       ps->nstages = factorize(ps->nf, ps->radix);
       if (ps->nstages < 0) { pa_error("cannot factor length %lld", (long long)ps->nf); goto fail; }
-      ps->tw = make_twiddles(p, ps->nf, ps->nf, sign);
-      p->tw_adj[k] = make_twiddles(p, ps->nf, ps->nf, -sign);
+      ps->tw = make_twiddles(p, ps->nf, ps->nf, ps->sign);
+      p->tw_adj[k] = make_twiddles(p, ps->nf, ps->nf, -ps->sign);
       if (!ps->tw || !p->tw_adj[k]) goto fail;
       if ((kind == PFK_R2C || kind == PFK_C2R) && n[d] % 2 == 0) {
         ps->rtw = make_twiddles(p, n[d], n[d] / 2 + 1, sign);
@@ -496,7 +532,7 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     cur = nxt;
   }
   {
-    const size_t cesz = (prec == PFK_F64) ? 16 : 8;
+    const size_t cesz = (prec == PFK_F64 ? 16 : 8) / (is_r2r ? 2 : 1);
     for (int i = 0; i < 2; i++) {
       p->scratch_bytes[i] = (size_t)scratch_elems[i] * cesz;
       if (p->scratch_bytes[i] && pfk_malloc(&p->scratch[i], p->scratch_bytes[i])) {
@@ -529,6 +565,10 @@ void *pa_plan_out(const pa_plan *p) { return p->out; }
 /* Runs the passes; `adjoint` runs them in reverse with swapped roles. */
 static void run(const pa_plan *p, void *in, void *out, int adjoint) {
   if (!p) return;
+  if (adjoint && p->is_r2r) {
+    pa_error("the adjoint pipeline is not available for r2r plans");
+    return;
+  }
   const size_t in_bytes = (size_t)(adjoint ? p->out_elems * p->out_esz : p->in_elems * p->in_esz);
   const size_t out_bytes = (size_t)(adjoint ? p->in_elems * p->in_esz : p->out_elems * p->out_esz);
   void *din = in, *dout = out, *stage_in = NULL, *stage_out = NULL;

@@ -7,8 +7,9 @@ from oracle import layout as L
 from oracle import semantics as S
 
 
-def spec_of(kind, n, ni=None, no=None, howmany=1, sign=-1, flags=0, rnk_pm=2, skip=None):
-    return S.PlanSpec(kind, n, ni, no, howmany, sign, flags, rnk_pm, skip)
+def spec_of(kind, n, ni=None, no=None, howmany=1, sign=-1, flags=0, rnk_pm=2, skip=None,
+            kinds=None):
+    return S.PlanSpec(kind, n, ni, no, howmany, sign, flags, rnk_pm, skip, kinds)
 
 
 def comm_for(rnk_pm):
@@ -48,7 +49,8 @@ def run_gpu(spec, x, prec="f64", comm=None, adjoint=False, plan_flags_extra=0, i
         comm = comm_for(spec.rnk_pm)
     skip = spec.skip if any(spec.skip) else None
     plan = P.Plan(spec.kind, spec.n, din.ptr, dout.ptr, spec.ni, spec.no, spec.howmany,
-                  spec.sign, spec.flags | plan_flags_extra, comm=comm, skip=skip, prec=prec)
+                  spec.sign, spec.flags | plan_flags_extra, comm=comm, skip=skip, prec=prec,
+                  kinds=spec.kinds)
     assert plan.ok, "plan creation failed"
     din.upload(np.asarray(x, dtype=rdt))
     if adjoint:

@@ -15,7 +15,7 @@ from oracle import layout as L
 
 TI, TO, SI, SO = P.TRANSPOSED_IN, P.TRANSPOSED_OUT, P.SHIFTED_IN, P.SHIFTED_OUT
 PAD = P.PADDED_R2C
-TF = {"c2c": L.T_C2C, "r2c": L.T_R2C, "c2r": L.T_C2R}
+TF = {"c2c": L.T_C2C, "r2c": L.T_R2C, "c2r": L.T_C2R, "r2r": L.T_R2R}
 
 
 def comms():
@@ -40,12 +40,12 @@ SIZES = [
 
 
 def cases():
-    for kind in ("c2c", "r2c", "c2r"):
+    for kind in ("c2c", "r2c", "c2r", "r2r"):
         for n, ni, no in SIZES:
             for flags in range(16):
                 if flags & 3 == 3:
                     continue
-                for pad in ((0, PAD) if kind != "c2c" else (0,)):
+                for pad in ((0, PAD) if kind in ("r2c", "c2r") else (0,)):
                     yield kind, n, ni or n, no or n, flags | pad
 
 
