@@ -9,6 +9,8 @@ cfg4  384 3-d c2c complex64 transforms, sides drawn from
       level, replayed hipGraph) vs a per-transform execute loop.
 cfg5  oversampled shifted r2c ni=128^3 -> n=256^3, loss sum w_k |X_k|^2:
       forward + adjoint (gradient) time, FP64 and FP32.
+gcell ghost-cell exchange and reduce of a 512^3 block with 4 ghost cells on
+      every side (FP64 real, FP64 complex r2c array, FP32 real).
 Each prints one JSON line.  Inputs are generated on the GPU with torch
 (test-data plumbing) and resident in HBM before timing.
 """
@@ -192,5 +194,52 @@ def cfg5(args):
     print(json.dumps(out))
 
 
+def gcell(args):
+    """pfft_exchange / pfft_reduce on a 512^3 block with 4 ghost cells per
+    side.  Minimal HBM bytes: exchange reads the block and writes the box
+    (L + G reals), reduce reads the box and writes the block plus the zeroed
+    rest of the box (2 G reals)."""
+    import torch
+    dev = torch.device("cuda:0")
+    rc, comm = P.create_procmesh([1, 1])
+    n, g = [512, 512, 512], 4
+    out = {"workload": "ghost cells 512^3 block, 4 cells each side", "cases": []}
+    for kind, prec, flags in (("r", "f64", 0), ("c", "f64", P.GC_R2C), ("r", "f32", 0)):
+        rdt = torch.float64 if prec == "f64" else torch.float32
+        pn = list(n)
+        tup = 1
+        if kind == "c":
+            pn[2] = n[2] // 2 + 1
+            tup = 2
+        L = tup * int(np.prod(pn))
+        G = tup * int(np.prod([v + 2 * g for v in pn]))
+        buf = torch.randn(G, device=dev, dtype=rdt)
+        plan = P.GcPlan(kind, n, buf.data_ptr(), [g] * 3, [g] * 3, flags=flags, comm=comm,
+                        prec=prec)
+        assert plan.ok
+        ms_x = _time(plan.exchange, 10, warm=2)
+        ms_r = _time(plan.reduce, 10, warm=2)
+        getattr(P.lib(), plan.P + "reset_gctimers")(plan.handle)
+        for _ in range(5):
+            plan.exchange()
+            plan.reduce()
+        (xi, xw, xp, xe), (ri, rw, rp, re_) = plan.timers()
+        esz = 8 if prec == "f64" else 4
+        out["cases"].append({
+            "data": "%s %s%s" % ("real" if kind == "r" else "complex", prec,
+                                  " r2c" if flags else ""),
+            "block_reals": L, "box_reals": G,
+            "ms_exchange": round(ms_x, 3), "ms_reduce": round(ms_r, 3),
+            "GBps_exchange_min_bytes": round(esz * (L + G) / (ms_x * 1e-3) / 1e9, 1),
+            "GBps_reduce_min_bytes": round(esz * 2 * G / (ms_r * 1e-3) / 1e9, 1),
+            "device_ms_exchange": {"block_copy": round(1e3 * xp / xi, 3),
+                                   "fill": round(1e3 * xe / xi, 3)},
+            "device_ms_reduce": {"sum": round(1e3 * re_ / ri, 3),
+                                 "compact": round(1e3 * rp / ri, 3)}})
+        plan.destroy()
+        del buf
+    print(json.dumps(out))
+
+
 def run(args):
-    {"cfg3": cfg3, "cfg4": cfg4, "cfg5": cfg5}[args.workload](args)
+    {"cfg3": cfg3, "cfg4": cfg4, "cfg5": cfg5, "gcell": gcell}[args.workload](args)

@@ -80,12 +80,27 @@ typedef float pfftf_complex[2];
 #define PFFT_DEFAULT_BLOCKS       NULL
 #define PFFT_NO_GCELLS            NULL
 
+/* Ghost-cell planner flags: same bit values as the reference
+ * (api/pfft.h:563-572).  PFFT_GC_SENDRECV and PFFT_GC_RMA choose an MPI
+ * algorithm in the reference and are accepted and ignored here. */
+#define PFFT_GC_TRANSPOSED_NONE   (0U)
+#define PFFT_GC_TRANSPOSED        (1U<< 0)
+#define PFFT_GC_SENDRECV          (1U<< 1)
+#define PFFT_GC_RMA               (1U<< 2)
+#define PFFT_GC_R2C               (1U<< 3)
+#define PFFT_GC_PADDED            (1U<< 4)
+#define PFFT_GC_C2R               (PFFT_GC_R2C)
+#define PFFT_GC_PADDED_R2C        (PFFT_GC_R2C | PFFT_GC_PADDED)
+#define PFFT_GC_PADDED_C2R        (PFFT_GC_C2R | PFFT_GC_PADDED)
+
 /*
  * One declaration list for both precisions.
  *   P: name prefix   R: real type   C: complex type   INT: ptrdiff_t
  */
 #define PFFT_AMD_DECLARE_API(P, R, C, INT)                                          \
   typedef struct P##plan_s *P##plan;                                                \
+  typedef struct P##gcplan_s *P##gcplan;                                            \
+  typedef struct P##gctimer_s *P##gctimer;                                          \
   /* setup: api/pfft.h:77-78 (init, cleanup), 79-80 (threads) */                   \
   void P##init(void);                                                               \
   void P##cleanup(void);                                                            \
@@ -255,7 +270,7 @@ typedef float pfftf_complex[2];
       const int *skip_trafos, C *in, R *out, MPI_Comm comm_cart, int sign,          \
       unsigned pfft_flags);                                                         \
   /* r2r planning: api/pfft.h:286-288, 299-301, 318-322, 339-343.  Kinds:        \
-   * PFFT_DHT, PFFT_REDFT00..11, PFFT_RODFT00..11 (no SHIFTED flags) */             \
+   * PFFT_DHT, PFFT_REDFT00..11, PFFT_RODFT00..11 */                                \
   P##plan P##plan_r2r_3d(const INT *n, R *in, R *out, MPI_Comm comm_cart,           \
       const P##r2r_kind *kinds, unsigned pfft_flags);                               \
   P##plan P##plan_r2r(int rnk_n, const INT *n, R *in, R *out, MPI_Comm comm_cart,   \
@@ -283,7 +298,52 @@ typedef float pfftf_complex[2];
       MPI_Comm *comm_cart);                                                         \
   int P##create_procmesh_1d(MPI_Comm comm, int np0, MPI_Comm *comm_cart_1d);        \
   int P##create_procmesh_2d(MPI_Comm comm, int np0, int np1,                        \
-      MPI_Comm *comm_cart_2d);
+      MPI_Comm *comm_cart_2d);                                                      \
+  /* ghost cells: api/pfft.h:433-501.  Three-dimensional arrays; on one         \
+   * process the ghost cells are periodic images of the local block.  The       \
+   * array must hold local_size_*gc reals / complex values. */                  \
+  INT P##local_size_gc_3d(const INT *local_n, const INT *local_start,              \
+      const INT *gc_below, const INT *gc_above, INT *local_ngc,                     \
+      INT *local_gc_start);                                                         \
+  INT P##local_size_gc(int rnk_n, const INT *local_n, const INT *local_start,      \
+      const INT *gc_below, const INT *gc_above, INT *local_ngc,                     \
+      INT *local_gc_start);                                                         \
+  INT P##local_size_many_gc(int rnk_n, const INT *local_n,                         \
+      const INT *local_start, INT howmany, const INT *gc_below,                     \
+      const INT *gc_above, INT *local_ngc, INT *local_gc_start);                    \
+  P##gcplan P##plan_rgc_3d(const INT *n, const INT *gc_below,                       \
+      const INT *gc_above, R *data, MPI_Comm comm_cart, unsigned gc_flags);         \
+  P##gcplan P##plan_cgc_3d(const INT *n, const INT *gc_below,                       \
+      const INT *gc_above, C *data, MPI_Comm comm_cart, unsigned gc_flags);         \
+  P##gcplan P##plan_rgc(int rnk_n, const INT *n, const INT *gc_below,               \
+      const INT *gc_above, R *data, MPI_Comm comm_cart, unsigned gc_flags);         \
+  P##gcplan P##plan_cgc(int rnk_n, const INT *n, const INT *gc_below,               \
+      const INT *gc_above, C *data, MPI_Comm comm_cart, unsigned gc_flags);         \
+  P##gcplan P##plan_many_rgc(int rnk_n, const INT *n, INT howmany,                  \
+      const INT *block, const INT *gc_below, const INT *gc_above, R *data,          \
+      MPI_Comm comm_cart, unsigned gc_flags);                                       \
+  P##gcplan P##plan_many_cgc(int rnk_n, const INT *n, INT howmany,                  \
+      const INT *block, const INT *gc_below, const INT *gc_above, C *data,          \
+      MPI_Comm comm_cart, unsigned gc_flags);                                       \
+  void P##exchange(P##gcplan ths);                                                  \
+  void P##reduce(P##gcplan ths);                                                    \
+  void P##destroy_gcplan(P##gcplan ths);                                            \
+  void P##reset_gctimers(P##gcplan ths);                                            \
+  P##gctimer P##get_gctimer_exg(const P##gcplan ths);                               \
+  P##gctimer P##get_gctimer_red(const P##gcplan ths);                               \
+  void P##print_average_gctimer(const P##gcplan ths, MPI_Comm comm);                \
+  void P##print_average_gctimer_adv(const P##gcplan ths, MPI_Comm comm);            \
+  void P##write_average_gctimer(const P##gcplan ths, const char *name,              \
+      MPI_Comm comm);                                                               \
+  void P##write_average_gctimer_adv(const P##gcplan ths, const char *name,          \
+      MPI_Comm comm);                                                               \
+  P##gctimer P##copy_gctimer(const P##gctimer orig);                                \
+  void P##average_gctimer(P##gctimer ths);                                          \
+  P##gctimer P##add_gctimers(const P##gctimer sum1, const P##gctimer sum2);         \
+  P##gctimer P##reduce_max_gctimer(const P##gctimer ths, MPI_Comm comm);            \
+  void P##convert_gctimer2vec(const P##gctimer ths, double *times);                 \
+  P##gctimer P##convert_vec2gctimer(const double *times);                           \
+  void P##destroy_gctimer(P##gctimer ths);
 
 PFFT_AMD_DECLARE_API(pfft_, double, pfft_complex, ptrdiff_t)
 PFFT_AMD_DECLARE_API(pfftf_, float, pfftf_complex, ptrdiff_t)

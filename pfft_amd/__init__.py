@@ -40,6 +40,16 @@ AMD_MFMA_F32 = 1 << 12
 AMD_MFMA_BF16X2 = 1 << 13
 FORWARD = -1
 BACKWARD = +1
+# ghost-cell flags: api/pfft.h:563-572
+GC_TRANSPOSED_NONE = 0
+GC_TRANSPOSED = 1 << 0
+GC_SENDRECV = 1 << 1
+GC_RMA = 1 << 2
+GC_R2C = 1 << 3
+GC_PADDED = 1 << 4
+GC_C2R = GC_R2C
+GC_PADDED_R2C = GC_R2C | GC_PADDED
+GC_PADDED_C2R = GC_PADDED_R2C
 DEFAULT_BLOCK = 0
 COMM_WORLD = 0x44000000
 
@@ -144,6 +154,29 @@ def _declare(L):
             f = getattr(L, P + nm)
             f.restype = ctypes.c_double if P == "pfft_" else ctypes.c_float
             f.argtypes = [ctypes.c_int, INTP, INTP, INTP, VP, ctypes.c_int]
+        f = getattr(L, P + "local_size_many_gc")
+        f.restype = INT
+        f.argtypes = [ctypes.c_int, INTP, INTP, INT, INTP, INTP, INTP, INTP]
+        for nm in ("plan_many_rgc", "plan_many_cgc"):
+            f = getattr(L, P + nm)
+            f.restype = VP
+            f.argtypes = [ctypes.c_int, INTP, INT, INTP, INTP, INTP, VP, ctypes.c_int,
+                          ctypes.c_uint]
+        for nm in ("exchange", "reduce", "destroy_gcplan", "reset_gctimers", "destroy_gctimer",
+                   "average_gctimer"):
+            getattr(L, P + nm).argtypes = [VP]
+            getattr(L, P + nm).restype = None
+        for nm in ("get_gctimer_exg", "get_gctimer_red", "copy_gctimer"):
+            getattr(L, P + nm).argtypes = [VP]
+            getattr(L, P + nm).restype = VP
+        getattr(L, P + "convert_gctimer2vec").argtypes = [VP, ctypes.POINTER(ctypes.c_double)]
+        getattr(L, P + "convert_gctimer2vec").restype = None
+        for nm in ("print_average_gctimer", "print_average_gctimer_adv"):
+            getattr(L, P + nm).argtypes = [VP, ctypes.c_int]
+            getattr(L, P + nm).restype = None
+        for nm in ("write_average_gctimer", "write_average_gctimer_adv"):
+            getattr(L, P + nm).argtypes = [VP, ctypes.c_char_p, ctypes.c_int]
+            getattr(L, P + nm).restype = None
     L.pfft_amd_memcpy_to_device.argtypes = [VP, VP, ctypes.c_size_t]
     L.pfft_amd_memcpy_to_device.restype = ctypes.c_int
     L.pfft_amd_memcpy_to_host.argtypes = [VP, VP, ctypes.c_size_t]
@@ -370,3 +403,53 @@ def execute_batch(plans):
         return
     arr = (VP * len(plans))(*[p.handle for p in plans])
     getattr(lib(), plans[0].P + "execute_batch")(arr, len(plans))
+
+
+# ---------------------------------------------------------------- ghost cells
+def local_size_gc(local_n, local_start, gc_below=None, gc_above=None, howmany=1, prec="f64"):
+    """pfft_local_size_many_gc: returns (mem, local_ngc, local_gc_start)."""
+    rnk = len(local_n)
+    lngc, lgcs = (INT * rnk)(), (INT * rnk)()
+    mem = getattr(lib(), _prefix(prec) + "local_size_many_gc")(
+        rnk, ivec(local_n), ivec(local_start), howmany, ivec(gc_below), ivec(gc_above),
+        lngc, lgcs)
+    return int(mem), list(lngc), list(lgcs)
+
+
+class GcPlan:
+    """A ghost-cell plan (pfft_plan_many_rgc / pfft_plan_many_cgc) on
+    `data` (real data: kind "r", complex data: kind "c")."""
+
+    def __init__(self, kind, n, data_ptr, gc_below=None, gc_above=None, howmany=1, block=None,
+                 flags=0, comm=COMM_WORLD, prec="f64"):
+        self.P = _prefix(prec)
+        name = {"r": "plan_many_rgc", "c": "plan_many_cgc"}[kind]
+        self.handle = getattr(lib(), self.P + name)(
+            len(n), ivec(n), howmany, ivec(block), ivec(gc_below), ivec(gc_above), data_ptr,
+            comm, flags)
+
+    @property
+    def ok(self):
+        return bool(self.handle)
+
+    def exchange(self):
+        getattr(lib(), self.P + "exchange")(self.handle)
+
+    def reduce(self):
+        getattr(lib(), self.P + "reduce")(self.handle)
+
+    def timers(self):
+        """(exchange, reduce) timers as [iter, whole, pad_zeros, exchange] lists."""
+        out = []
+        for nm in ("get_gctimer_exg", "get_gctimer_red"):
+            t = getattr(lib(), self.P + nm)(self.handle)
+            v = (ctypes.c_double * 4)()
+            getattr(lib(), self.P + "convert_gctimer2vec")(t, v)
+            getattr(lib(), self.P + "destroy_gctimer")(t)
+            out.append(list(v))
+        return out
+
+    def destroy(self):
+        if self.handle:
+            getattr(lib(), self.P + "destroy_gcplan")(self.handle)
+            self.handle = None

@@ -431,3 +431,175 @@ R PA(check_output_real_3d)(const INT *n, const INT *ln, const INT *ls, const R *
                            MPI_Comm comm) {
   return PA(check_output_real)(3, n, ln, ls, data, comm);
 }
+
+/* ---------------- ghost cells: api/api-basic.c:482-590, api/api-adv.c:223-326 ---------------- */
+INT PA(local_size_many_gc)(int rnk_n, const INT *local_n, const INT *local_start, INT howmany,
+                           const INT *gc_below, const INT *gc_above, INT *local_ngc,
+                           INT *local_gc_start) {
+  return pa_local_size_gc(rnk_n, local_n, local_start, howmany, gc_below, gc_above, local_ngc,
+                          local_gc_start);
+}
+INT PA(local_size_gc)(int rnk_n, const INT *local_n, const INT *local_start,
+                      const INT *gc_below, const INT *gc_above, INT *local_ngc,
+                      INT *local_gc_start) {
+  return PA(local_size_many_gc)(rnk_n, local_n, local_start, 1, gc_below, gc_above, local_ngc,
+                                local_gc_start);
+}
+INT PA(local_size_gc_3d)(const INT *local_n, const INT *local_start, const INT *gc_below,
+                         const INT *gc_above, INT *local_ngc, INT *local_gc_start) {
+  return PA(local_size_gc)(3, local_n, local_start, gc_below, gc_above, local_ngc,
+                           local_gc_start);
+}
+PA(gcplan) PA(plan_many_rgc)(int rnk_n, const INT *n, INT howmany, const INT *block,
+                             const INT *gc_below, const INT *gc_above, R *data,
+                             MPI_Comm comm_cart, unsigned gc_flags) {
+  return (PA(gcplan))pa_gcplan_create(PA_PREC, rnk_n, n, howmany, block, gc_below, gc_above,
+                                      data, comm_cart, gc_flags, 0);
+}
+PA(gcplan) PA(plan_many_cgc)(int rnk_n, const INT *n, INT howmany, const INT *block,
+                             const INT *gc_below, const INT *gc_above, CPLX *data,
+                             MPI_Comm comm_cart, unsigned gc_flags) {
+  return (PA(gcplan))pa_gcplan_create(PA_PREC, rnk_n, n, howmany, block, gc_below, gc_above,
+                                      data, comm_cart, gc_flags, 1);
+}
+PA(gcplan) PA(plan_rgc)(int rnk_n, const INT *n, const INT *gc_below, const INT *gc_above,
+                        R *data, MPI_Comm comm_cart, unsigned gc_flags) {
+  return PA(plan_many_rgc)(rnk_n, n, 1, PFFT_DEFAULT_BLOCKS, gc_below, gc_above, data,
+                           comm_cart, gc_flags);
+}
+PA(gcplan) PA(plan_cgc)(int rnk_n, const INT *n, const INT *gc_below, const INT *gc_above,
+                        CPLX *data, MPI_Comm comm_cart, unsigned gc_flags) {
+  return PA(plan_many_cgc)(rnk_n, n, 1, PFFT_DEFAULT_BLOCKS, gc_below, gc_above, data,
+                           comm_cart, gc_flags);
+}
+PA(gcplan) PA(plan_rgc_3d)(const INT *n, const INT *gc_below, const INT *gc_above, R *data,
+                           MPI_Comm comm_cart, unsigned gc_flags) {
+  return PA(plan_rgc)(3, n, gc_below, gc_above, data, comm_cart, gc_flags);
+}
+PA(gcplan) PA(plan_cgc_3d)(const INT *n, const INT *gc_below, const INT *gc_above, CPLX *data,
+                           MPI_Comm comm_cart, unsigned gc_flags) {
+  return PA(plan_cgc)(3, n, gc_below, gc_above, data, comm_cart, gc_flags);
+}
+/* a NULL plan (zero ghost cells) does nothing: api-basic.c:556-578 */
+void PA(exchange)(PA(gcplan) ths) {
+  if (ths) pa_gc_exchange((pa_gcplan *)ths);
+}
+void PA(reduce)(PA(gcplan) ths) {
+  if (ths) pa_gc_reduce((pa_gcplan *)ths);
+}
+void PA(destroy_gcplan)(PA(gcplan) ths) { pa_gcplan_destroy((pa_gcplan *)ths); }
+
+/* ---------------- ghost-cell timers: gcell/gctimer.c ---------------- */
+static PA(gctimer) PA(gctimer_new)(const pa_gctimer *src) {
+  pa_gctimer *t = (pa_gctimer *)calloc(1, sizeof(pa_gctimer));
+  if (t && src) *t = *src;
+  return (PA(gctimer))t;
+}
+void PA(reset_gctimers)(PA(gcplan) ths) {
+  if (!ths) return;
+  memset(pa_gcplan_timer((pa_gcplan *)ths, 0), 0, sizeof(pa_gctimer));
+  memset(pa_gcplan_timer((pa_gcplan *)ths, 1), 0, sizeof(pa_gctimer));
+}
+PA(gctimer) PA(get_gctimer_exg)(const PA(gcplan) ths) {
+  return ths ? PA(gctimer_new)(pa_gcplan_timer((pa_gcplan *)ths, 0)) : NULL;
+}
+PA(gctimer) PA(get_gctimer_red)(const PA(gcplan) ths) {
+  return ths ? PA(gctimer_new)(pa_gcplan_timer((pa_gcplan *)ths, 1)) : NULL;
+}
+PA(gctimer) PA(copy_gctimer)(const PA(gctimer) orig) {
+  return PA(gctimer_new)((const pa_gctimer *)orig);
+}
+void PA(average_gctimer)(PA(gctimer) ths) {
+  pa_gctimer *t = (pa_gctimer *)ths;
+  if (!t || t->iter <= 0) return;
+  t->whole /= t->iter;
+  t->pad_zeros /= t->iter;
+  t->exchange /= t->iter;
+  t->iter = 1;
+}
+PA(gctimer) PA(add_gctimers)(const PA(gctimer) sum1, const PA(gctimer) sum2) {
+  pa_gctimer *t = (pa_gctimer *)PA(copy_gctimer)(sum1);
+  const pa_gctimer *b = (const pa_gctimer *)sum2;
+  if (t && b) {
+    t->iter += b->iter;
+    t->whole += b->whole;
+    t->pad_zeros += b->pad_zeros;
+    t->exchange += b->exchange;
+  }
+  return (PA(gctimer))t;
+}
+/* one process: the maximum over ranks is the timer itself */
+PA(gctimer) PA(reduce_max_gctimer)(const PA(gctimer) ths, MPI_Comm comm) {
+  (void)comm;
+  return PA(copy_gctimer)(ths);
+}
+void PA(convert_gctimer2vec)(const PA(gctimer) ths, double *times) {
+  const pa_gctimer *t = (const pa_gctimer *)ths;
+  times[0] = (double)t->iter;
+  times[1] = t->whole;
+  times[2] = t->pad_zeros;
+  times[3] = t->exchange;
+}
+PA(gctimer) PA(convert_vec2gctimer)(const double *times) {
+  pa_gctimer t = {(int)times[0], times[1], times[2], times[3]};
+  return PA(gctimer_new)(&t);
+}
+void PA(destroy_gctimer)(PA(gctimer) ths) { free(ths); }
+
+/* output format of gcell/gctimer.c:92-145, 274-294 (Matlab assignments;
+ * index 1 = log2(1 process) + 1) */
+static void PA(gc_print)(const PA(gcplan) ths, MPI_Comm comm, FILE *f, int header, int adv) {
+  INT n[3], gb[3], ga[3];
+  pa_gcplan_describe((const pa_gcplan *)ths, n, gb, ga);
+  if (header) {
+    PA(fprintf)(comm, f, "\n%% n - FFT size\n");
+    PA(fprintf)(comm, f, "%% np - process grid\n");
+    PA(fprintf)(comm, f, "%% procs - number of processes\n");
+    PA(fprintf)(comm, f, "%% index(i) = log(procs(i)) + 1\n");
+  }
+  PA(fprintf)(comm, f, "\nindex(%d) = %d;  procs(%d) = %d;  ", 1, 1, 1, 1);
+  PA(fprintf)(comm, f, "np_gc(%d, 1:3) = [%d %d %d];\n", 1, 1, 1, 1);
+  PA(fprintf)(comm, f, "n_gc(%d, 1:3) = [%td %td %td];  ", 1, n[0], n[1], n[2]);
+  PA(fprintf)(comm, f, "gc_below(%d, 1:3) = [%td %td %td];  ", 1, gb[0], gb[1], gb[2]);
+  PA(fprintf)(comm, f, "gc_above(%d, 1:3) = [%td %td %td];\n", 1, ga[0], ga[1], ga[2]);
+  for (int pass = 0; pass < (adv ? 2 : 1); pass++) {
+    for (int w = 0; w < 2; w++) {
+      const char *prefix = w ? "gcells_red" : "gcells_exg";
+      const pa_gctimer *t = pa_gcplan_timer((pa_gcplan *)ths, w);
+      pa_gctimer m = *t;
+      PA(average_gctimer)((PA(gctimer))&m);
+      if (pass == 0) {
+        PA(fprintf)(comm, f, "%s_gc_iter(%d)      = %d;  ", prefix, 1, t->iter);
+        PA(fprintf)(comm, f, "%s_gcells(%d)       = %.3e;\n", prefix, 1, m.whole);
+      } else {
+        PA(fprintf)(comm, f, "%s_gc_pad_zeros(%d) = %.3e;  ", prefix, 1, m.pad_zeros);
+        PA(fprintf)(comm, f, "%s_gc_exchange(%d)  = %.3e;\n", prefix, 1, m.exchange);
+      }
+    }
+  }
+}
+void PA(print_average_gctimer)(const PA(gcplan) ths, MPI_Comm comm) {
+  if (ths) PA(gc_print)(ths, comm, stdout, 1, 0);
+}
+void PA(print_average_gctimer_adv)(const PA(gcplan) ths, MPI_Comm comm) {
+  if (ths) PA(gc_print)(ths, comm, stdout, 1, 1);
+}
+static void PA(gc_write)(const PA(gcplan) ths, const char *name, MPI_Comm comm, int adv) {
+  if (!ths || !name) return;
+  FILE *probe = fopen(name, "r");
+  const int newfile = (probe == NULL);
+  if (probe) fclose(probe);
+  FILE *f = fopen(name, "a+");
+  if (!f) {
+    PA(fprintf)(comm, stderr, "Error: Cannot open file %s.\n", name);
+    return;
+  }
+  PA(gc_print)(ths, comm, f, newfile, adv);
+  fclose(f);
+}
+void PA(write_average_gctimer)(const PA(gcplan) ths, const char *name, MPI_Comm comm) {
+  PA(gc_write)(ths, name, comm, 0);
+}
+void PA(write_average_gctimer_adv)(const PA(gcplan) ths, const char *name, MPI_Comm comm) {
+  PA(gc_write)(ths, name, comm, 1);
+}

@@ -77,6 +77,25 @@ void pa_wisdom_forget(void);
 int pa_wisdom_count(void);
 int pa_tune_timings(void);
 
+/* ---- ghost cells (gcell.c) ---- */
+/* exchange / reduce timer: gcell/gctimer.c fields (seconds) */
+typedef struct pa_gctimer {
+  int iter;
+  double whole, pad_zeros, exchange;
+} pa_gctimer;
+typedef struct pa_gcplan pa_gcplan;
+INT pa_local_size_gc(int rnk_n, const INT *local_n, const INT *local_start, INT howmany,
+                     const INT *gc_below, const INT *gc_above, INT *local_ngc,
+                     INT *local_gc_start);
+pa_gcplan *pa_gcplan_create(int prec, int rnk_n, const INT *n, INT howmany, const INT *block,
+                            const INT *gc_below, const INT *gc_above, void *data,
+                            MPI_Comm comm, unsigned gc_flags, int complex_data);
+void pa_gc_exchange(pa_gcplan *p);
+void pa_gc_reduce(pa_gcplan *p);
+void pa_gcplan_destroy(pa_gcplan *p);
+pa_gctimer *pa_gcplan_timer(pa_gcplan *p, int which);   /* 0 exchange, 1 reduce */
+void pa_gcplan_describe(const pa_gcplan *p, INT *n, INT *gb, INT *ga);
+
 /* ---- twiddles (twiddle.c) ---- */
 /* exp(sign*2*pi*i*m/N) for m in [0, count), rounded to double, computed in
  * long double with exact integer quadrant reduction. */

@@ -125,6 +125,26 @@ pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_graph);
 int  pfk_batch_run(pfk_batch *b);
 void pfk_batch_destroy(pfk_batch *b);
 
+/* Ghost cells of a 3-d local block on one process (sizes in the plan's
+ * memory order, tuple reals innermost).  The block n[0] x n[1] x n[2] is
+ * periodic with period n[t] along dimension t; the box extends it by
+ * gb[t] cells below and ngc[t] - gb[t] - n[t] above.
+ *   exchange: the block at the front of `data` becomes the box, every
+ *             ghost cell holding its periodic image;
+ *   reduce:   every block element receives the sum of its images in the
+ *             box `data`, the block is stored at the front of `data` and
+ *             the rest of the box is zeroed.
+ * `scratch` holds tuple*n0*n1*n2 reals.  `ev` (may be NULL) holds three
+ * events recorded before, between and after the two phases. */
+typedef struct pfk_gc {
+  int32_t prec;
+  int32_t pad;
+  int64_t n[3], gb[3], ngc[3];
+  int64_t tuple;
+} pfk_gc;
+int pfk_gc_exchange(const pfk_gc *g, void *data, void *scratch, void *const *ev);
+int pfk_gc_reduce(const pfk_gc *g, void *data, void *scratch, void *const *ev);
+
 /* Device/runtime plumbing.  All return 0 on success, nonzero on error. */
 int  pfk_init(void);
 int  pfk_device_count(void);
