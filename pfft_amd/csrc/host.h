@@ -100,6 +100,8 @@ void pa_gcplan_describe(const pa_gcplan *p, INT *n, INT *gb, INT *ga);
 /* exp(sign*2*pi*i*m/N) for m in [0, count), rounded to double, computed in
  * long double with exact integer quadrant reduction. */
 void pa_twiddles(long N, long count, int sign, double *out_interleaved);
+/* the same for m = step*j, j in [0, count) */
+void pa_twiddles_step(long N, long count, long step, int sign, double *out_interleaved);
 
 /* ---- errors ---- */
 void pa_error(const char *fmt, ...);

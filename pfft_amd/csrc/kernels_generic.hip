@@ -118,6 +118,88 @@ int launch_generic(const pfk_pass *p, const void *src, void *dst) {
                               : launch_generic_t<float>(p, src, dst);
 }
 
+// A long c2c line (nf = n1 * n2 > PFK_MAXLINE) in two steps through the
+// plan's work buffer W, laid out [line][k1][j2] (line = tile index, then the
+// batch indices, innermost first):
+//   step A: for every line and j2, the length-n1 DFT of the decimated line
+//           x[j2], x[j2 + n2], ... (embed and input modulation applied to
+//           the original index), times W_nf^(j2*k1), stored at W[line][k1][j2];
+//   step B: for every line and k1, the length-n2 DFT over j2, stored at
+//           original output index k1 + n1*k2 (trunc and output modulation).
+// Both steps take j2 resp. k1 as the tile dimension (consecutive lines are
+// adjacent in memory when the line stride is 1).  Step A runs on the
+// power-of-two register kernels (twiddled-store instantiation) unless the
+// line is embedded or modulated; step B on any kernel unless it truncates
+// or modulates.
+static int launch_long(const pfk_pass *p, const void *src, void *dst) {
+  const int64_t N = p->nf, N1 = p->n1, N2 = N / N1;
+  if (p->kind != PFK_C2C || N1 < 2 || N1 * N2 != N || N1 > PFK_MAXLINE || N2 > PFK_MAXLINE ||
+      !p->work) {
+    set_error_msg("pfk: bad long-line pass");
+    return 1;
+  }
+  const int nbo = p->nb + (p->text > 1 ? 1 : 0);   // original line dims
+  if (nbo > PFK_MAXB) { set_error_msg("pfk: too many batch dimensions for a long line"); return 1; }
+  int64_t ext[PFK_MAXB + 1], osin[PFK_MAXB + 1], osout[PFK_MAXB + 1];
+  int m = 0;
+  if (p->text > 1) { ext[m] = p->text; osin[m] = p->tsin; osout[m] = p->tsout; m++; }
+  for (int i = 0; i < p->nb; i++) {
+    ext[m] = p->bext[i]; osin[m] = p->bsin[i]; osout[m] = p->bsout[i]; m++;
+  }
+  int64_t lines = 1;
+  for (int i = 0; i < m; i++) lines *= ext[i];
+  const size_t ce = (p->prec == PFK_F64) ? 16 : 8;
+  const char *tw = (const char *)p->tw;
+
+  pfk_pass a = *p;
+  a.n1 = 0; a.work = NULL; a.hw = 0; a.mfma = 0; a.dec_out = 0;
+  a.n = a.nf = N1;
+  a.nstages = pfk_factorize(N1, a.radix);
+  a.tw = tw;
+  a.twn = N;
+  a.tw2 = tw + ce * (size_t)(N1 + N2);
+  // embed / modulation of the original line: decimated loads (general
+  // kernel); otherwise plain strided lines (any kernel with the twiddle)
+  const int embed = p->nin != N || p->zin != 0 || p->imod.on;
+  a.dec_in = embed ? N2 : 0;
+  a.variant = embed ? 1 : 0;
+  if (!embed) { a.nin = N1; a.zin = 0; }
+  a.sin = N2 * p->sin;
+  a.nout = N1; a.zout = 0; a.omod.on = 0;
+  a.sout = N2;
+  a.text = N2; a.tsin = p->sin; a.tsout = 1;
+  a.nb = m;
+  int64_t wst = N;
+  for (int i = 0; i < m; i++) {
+    a.bext[i] = ext[i]; a.bsin[i] = osin[i]; a.bsout[i] = wst;
+    wst *= ext[i];
+  }
+  a.dst_elems = lines * N;
+
+  pfk_pass b = *p;
+  b.n1 = 0; b.work = NULL; b.hw = 0; b.dec_in = 0; b.twn = 0; b.tw2 = NULL;
+  b.n = b.nf = N2;
+  b.nstages = pfk_factorize(N2, b.radix);
+  b.tw = tw + ce * (size_t)N1;
+  b.nin = N2; b.zin = 0; b.imod.on = 0;
+  b.sin = 1;
+  const int trunc = p->nout != N || p->zout != 0 || p->omod.on;
+  b.dec_out = trunc ? N1 : 0;           // trunc / modulation of the original line
+  if (!trunc) { b.nout = N2; b.zout = 0; }
+  b.sout = N1 * p->sout;
+  b.text = N1; b.tsin = N2; b.tsout = p->sout;
+  b.nb = m;
+  wst = N;
+  for (int i = 0; i < m; i++) {
+    b.bext[i] = ext[i]; b.bsin[i] = wst; b.bsout[i] = osout[i];
+    wst *= ext[i];
+  }
+  b.src_elems = lines * N;
+  if (a.nstages < 0 || b.nstages < 0) { set_error_msg("pfk: long-line factors"); return 1; }
+  if (pfk_launch_pass(&a, src, p->work)) return 1;
+  return pfk_launch_pass(&b, p->work, dst);
+}
+
 template <typename T>
 __global__ void k_scale(T *d, int64_t n, T f) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -218,8 +300,17 @@ int pfk_sync(void) {
   return 0;
 }
 
+int pfk_pass_check(const pfk_pass *p) {
+  if (pfk_init()) return 1;
+  if (!g_generic_only && (fast_covers(p) || small_covers(p) || mfma_covers(p))) return 0;
+  int C, pitch;
+  size_t lds;
+  return generic_tiling(p, &C, &pitch, &lds);
+}
+
 int pfk_launch_pass(const pfk_pass *p, const void *src, void *dst) {
   if (pfk_init()) return 1;
+  if (p->n1) return launch_long(p, src, dst);
   if (!g_generic_only && p->mfma) {
     const int m = launch_mfma(p, src, dst);
     if (m == 0) return 0;

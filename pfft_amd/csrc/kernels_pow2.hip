@@ -253,6 +253,7 @@ __device__ __forceinline__ void fft_regs(C2<T> *v, C2<T> *lds, const int t, cons
 
 struct TileCtx {
   int64_t sb, db;
+  int64_t c0;   // first line of the tile along the tile dimension
   int nl;
 };
 
@@ -276,6 +277,7 @@ __device__ __forceinline__ TileCtx tile_at(const pfk_pass &P, int C, int64_t ite
   int64_t bflat = tileid / ntt;
   const int64_t c0 = tt * C;
   TileCtx r;
+  r.c0 = c0;
   r.nl = (int)((P.text - c0) < C ? (P.text - c0) : C);
   r.sb = c0 * P.tsin;
   r.db = c0 * P.tsout;
@@ -337,7 +339,9 @@ __device__ __forceinline__ void load_c2c(const pfk_pass &P, const T *src, const 
 // LM: line-major lane order (thread = c*T + t: a wave's lanes run along a
 // line), for passes whose lines are contiguous in memory; otherwise
 // thread = t*C + c (a wave's lanes run across the tile's lines).
-template <typename T, int NF, int C, int E, int SIGN, bool LM = false>
+// TW: output k of tile line c0 + c is multiplied by W_twn^((c0+c)*k) from
+// the two-level table tw2 (first step of a long line, pfk.h).
+template <typename T, int NF, int C, int E, int SIGN, bool LM = false, bool TW = false>
 __global__ void __launch_bounds__(C *NF / E)
 k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
            const T *__restrict__ src, T *__restrict__ dst) {
@@ -381,7 +385,13 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
         const int k = t + TT * m - zout;
         if (k >= 0 && k < nout) {
           const T f = msign<T>(P.omod, k);
-          st_stream(d + (int64_t)k * P.sout, C2<T>{cur[m].x * f, cur[m].y * f});
+          C2<T> v{cur[m].x * f, cur[m].y * f};
+          if constexpr (TW) {
+            const C2<T> *lo = reinterpret_cast<const C2<T> *>(P.tw2);
+            const int64_t mm = (tc.c0 + c) * (int64_t)(k + zout);
+            v = cmul(v, cmul(lo[mm & 4095], lo[4096 + (mm >> 12)]));
+          }
+          st_stream(d + (int64_t)k * P.sout, v);
         }
       }
     }
@@ -639,13 +649,21 @@ static int launch_items(void (*k)(pfk_pass, int64_t, int64_t, int64_t, const T *
   return 0;
 }
 
-template <typename T, int NF, int C, int E, int SIGN>
+template <typename T, int NF, int C, int E, int SIGN, bool TWOK>
 static int launch_kind(const pfk_pass *p, int64_t ntiles, const void *src, void *dst) {
   constexpr int NT = C * NF / E;
-  static int64_t grid_c2c = 0, grid_lm = 0, grid_r2c = 0, grid_c2r = 0;   // per instantiation
+  static int64_t grid_c2c = 0, grid_lm = 0, grid_r2c = 0, grid_c2r = 0, grid_tw = 0;
   const bool persistent = !(p->variant & 4);
   const T *s = (const T *)src;
   T *d = (T *)dst;
+  if (p->kind == PFK_C2C && p->twn) {
+    if constexpr (TWOK) {
+      if (!grid_tw) grid_tw = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, false, true>, NT);
+      return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, false, true>, p, ntiles, grid_tw,
+                          persistent, NT, s, d);
+    }
+    return 1;
+  }
   if (p->kind == PFK_C2C) {
     if constexpr (sizeof(T) == 4) {   // line-major lane order: complex64 only
       if (p->variant & 8) {
@@ -664,13 +682,14 @@ static int launch_kind(const pfk_pass *p, int64_t ntiles, const void *src, void 
   return launch_items(k_c2r_pow2<T, NF, C, E, SIGN>, p, ntiles, grid_c2r, persistent, NT, s, d);
 }
 
-template <typename T, int NF, int C, int E>
+// TWOK: the table also instantiates the twiddled first step of long lines
+template <typename T, int NF, int C, int E, bool TWOK = true>
 static int launch_nf(const pfk_pass *p, const void *src, void *dst) {
   int64_t ntiles = (p->text + C - 1) / C;
   for (int i = 0; i < p->nb; ++i) ntiles *= p->bext[i];
   if (ntiles <= 0) return 0;
-  return p->sign < 0 ? launch_kind<T, NF, C, E, -1>(p, ntiles, src, dst)
-                     : launch_kind<T, NF, C, E, +1>(p, ntiles, src, dst);
+  return p->sign < 0 ? launch_kind<T, NF, C, E, -1, TWOK>(p, ntiles, src, dst)
+                     : launch_kind<T, NF, C, E, +1, TWOK>(p, ntiles, src, dst);
 }
 
 
@@ -695,13 +714,13 @@ static int launch_pow2_f64(const pfk_pass *p, const void *src, void *dst) {
 // alternative (narrower) tiles: half the lines per workgroup, half the LDS
 static int launch_pow2_f64_alt(const pfk_pass *p, const void *src, void *dst) {
   switch (p->nf) {
-    case 64: return launch_nf<double, 64, 4, 8>(p, src, dst);
-    case 128: return launch_nf<double, 128, 4, 8>(p, src, dst);
-    case 256: return launch_nf<double, 256, 4, 8>(p, src, dst);
-    case 512: return launch_nf<double, 512, 4, 8>(p, src, dst);
-    case 1024: return launch_nf<double, 1024, 4, 16>(p, src, dst);
-    case 2048: return launch_nf<double, 2048, 2, 16>(p, src, dst);
-    case 4096: return launch_nf<double, 4096, 1, 16>(p, src, dst);
+    case 64: return launch_nf<double, 64, 4, 8, false>(p, src, dst);
+    case 128: return launch_nf<double, 128, 4, 8, false>(p, src, dst);
+    case 256: return launch_nf<double, 256, 4, 8, false>(p, src, dst);
+    case 512: return launch_nf<double, 512, 4, 8, false>(p, src, dst);
+    case 1024: return launch_nf<double, 1024, 4, 16, false>(p, src, dst);
+    case 2048: return launch_nf<double, 2048, 2, 16, false>(p, src, dst);
+    case 4096: return launch_nf<double, 4096, 1, 16, false>(p, src, dst);
     default: return 1;
   }
 }
@@ -721,18 +740,20 @@ static int launch_pow2_f32(const pfk_pass *p, const void *src, void *dst) {
 
 static int launch_pow2_f32_alt(const pfk_pass *p, const void *src, void *dst) {
   switch (p->nf) {
-    case 64: return launch_nf<float, 64, 8, 8>(p, src, dst);
-    case 128: return launch_nf<float, 128, 8, 8>(p, src, dst);
-    case 256: return launch_nf<float, 256, 8, 8>(p, src, dst);
-    case 512: return launch_nf<float, 512, 8, 8>(p, src, dst);
-    case 1024: return launch_nf<float, 1024, 8, 16>(p, src, dst);
-    case 2048: return launch_nf<float, 2048, 4, 16>(p, src, dst);
-    case 4096: return launch_nf<float, 4096, 2, 16>(p, src, dst);
+    case 64: return launch_nf<float, 64, 8, 8, false>(p, src, dst);
+    case 128: return launch_nf<float, 128, 8, 8, false>(p, src, dst);
+    case 256: return launch_nf<float, 256, 8, 8, false>(p, src, dst);
+    case 512: return launch_nf<float, 512, 8, 8, false>(p, src, dst);
+    case 1024: return launch_nf<float, 1024, 8, 16, false>(p, src, dst);
+    case 2048: return launch_nf<float, 2048, 4, 16, false>(p, src, dst);
+    case 4096: return launch_nf<float, 4096, 2, 16, false>(p, src, dst);
     default: return 1;
   }
 }
 
 int fast_covers(const pfk_pass *p) {
+  if (p->n1 || p->dec_in || p->dec_out) return 0;   // long lines, decimated steps
+  if (p->twn && (p->kind != PFK_C2C || (p->variant & ~4))) return 0;   // default tiles only
   if (p->kind != PFK_C2C && p->kind != PFK_R2C && p->kind != PFK_C2R) return 0;
   const int64_t nf = p->nf;
   if (nf < 64 || nf > 4096 || (nf & (nf - 1))) return 0;

@@ -13,6 +13,7 @@ k_small(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
 }
 
 int small_covers(const pfk_pass *p) {
+  if (p->n1 || p->dec_in || p->dec_out || p->twn) return 0;   // steps of long lines
   if (p->kind != PFK_C2C || p->hw || p->nf != p->n || p->n > 128) return 0;
   return small_length((int)p->nf, p->prec == PFK_F64) ? 1 : 0;
 }

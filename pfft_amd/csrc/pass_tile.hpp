@@ -67,7 +67,30 @@ __device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const 
   const bool st_tile_fast = (P.tsout == 1 && P.sout != 1);
 
   // ---------------- load (embed + input modulation) ----------------
-  if (KIND == PFK_C2C || KIND == PFK_COPY) {
+  if (KIND == PFK_C2C && P.dec_in) {
+    // first step of a long line: element j of line cc is index
+    // j*dec_in + c0 + cc of the original line (stride tsin), which carries
+    // the embed and the input modulation; elements outside the embed read
+    // element 0 and are zeroed
+    const C2 *s = reinterpret_cast<const C2 *>(src);
+    const int64_t line0 = sb - c0 * P.tsin;   // element 0 of the original line
+    batched<8, C2>(
+        tid, NT, nl * n,
+        [&](int idx) {
+          const int cc = ld_tile_fast ? idx % nl : idx / n;
+          const int j = ld_tile_fast ? idx / nl : idx % n;
+          const int64_t jo = (int64_t)j * P.dec_in + c0 + cc - zin;
+          const bool ok = jo >= 0 && jo < nin;
+          return s[line0 + (ok ? jo : 0) * P.tsin];
+        },
+        [&](int idx, C2 v) {
+          const int cc = ld_tile_fast ? idx % nl : idx / n;
+          const int j = ld_tile_fast ? idx / nl : idx % n;
+          const int64_t jo = (int64_t)j * P.dec_in + c0 + cc - zin;
+          const T f = (jo >= 0 && jo < nin) ? (T)mod_sign(P.imod, jo) : (T)0;
+          buf[cc * pitch + j] = C2{v.x * f, v.y * f};
+        });
+  } else if (KIND == PFK_C2C || KIND == PFK_COPY) {
     const C2 *s = reinterpret_cast<const C2 *>(src);
     if (nin < n)
       for (int idx = tid; idx < nl * n; idx += NT) {
@@ -290,7 +313,38 @@ __device__ __forceinline__ void pass_tile(const pfk_pass &P, const int C, const 
   }
 
   // ---------------- store (trunc + output modulation) ----------------
-  if (KIND == PFK_C2C || KIND == PFK_COPY) {
+  if (KIND == PFK_C2C && (P.dec_out || P.twn)) {
+    // steps of a long line.  twn: output k of line cc times
+    // W_twn^((c0+cc)*k) = lo[m % 4096] * hi[m / 4096] (m < twn).
+    // dec_out: output k of line cc is index k*dec_out + c0 + cc of the
+    // original output line (trunc + modulation; sout = dec_out * tsout).
+    C2 *d = reinterpret_cast<C2 *>(dst);
+    const C2 *lo = reinterpret_cast<const C2 *>(P.tw2);
+    const C2 *hi = lo + 4096;
+    const int cnt = P.dec_out ? n : nout;
+    const int z = P.dec_out ? 0 : zout;
+    const int64_t zoff = (int64_t)zout * P.tsout;
+    for (int idx = tid; idx < nl * cnt; idx += NT) {
+      int cc, k;
+      if (st_tile_fast) { cc = idx % nl; k = idx / nl; }
+      else { k = idx % cnt; cc = idx / cnt; }
+      C2 v = buf[cc * pitch + z + k];
+      if (P.twn) {
+        const int64_t m = (c0 + cc) * (int64_t)(z + k);
+        v = cmul(v, cmul(lo[m & 4095], hi[m >> 12]));
+      }
+      if (P.dec_out) {
+        const int64_t ko = (int64_t)k * P.dec_out + c0 + cc - zout;
+        if (ko >= 0 && ko < nout) {
+          const T f = (T)mod_sign(P.omod, ko);
+          d[db + cc * P.tsout + (int64_t)k * P.sout - zoff] = C2{v.x * f, v.y * f};
+        }
+      } else {
+        const T f = (T)mod_sign(P.omod, k);
+        d[db + cc * P.tsout + (int64_t)k * P.sout] = C2{v.x * f, v.y * f};
+      }
+    }
+  } else if (KIND == PFK_C2C || KIND == PFK_COPY) {
     C2 *d = reinterpret_cast<C2 *>(dst);
     for (int idx = tid; idx < nl * nout; idx += NT) {
       int cc, k;

@@ -93,6 +93,21 @@ typedef struct pfk_pass {
                          workgroup per tile instead of the persistent grid; + 8: line-major
                          lane order (complex64 c2c) */
   int32_t r2r;        /* enum pfk_r2r of a PFK_R2R pass */
+  /* Long lines (c2c, nf > PFK_MAXLINE): nf = n1 * n2 is computed in two
+   * steps through `work` (lines * nf complex elements, owned by the plan):
+   * length-n1 DFTs over the decimated lines j2, j2 + n2, ... times
+   * W_nf^(j2*k1), then length-n2 DFTs (Cooley-Tukey four-step).  The twiddle
+   * table holds W_n1 (n1), W_n2 (n2), W_nf^l (l < 4096) and W_nf^(4096 h). */
+  int64_t n1;         /* 0: the line fits one workgroup */
+  void *work;
+  /* Fields of the two steps (set by the long-line launcher; 0 otherwise):
+   * dec_in:  element j of line c is index j*dec_in + c of the original line,
+   *          whose embed (nin, zin) and input modulation apply;
+   * dec_out: output k of line c is index k*dec_out + c of the original
+   *          output line, whose trunc (nout, zout) and modulation apply;
+   * twn:     output k of line c is multiplied by W_twn^(c*k) (table tw2). */
+  int64_t dec_in, dec_out, twn;
+  const void *tw2;
   /* device twiddle tables (owned by the host plan):
    *   tw  : W_nf^m = exp(sign*2*pi*i*m/nf), m in [0,nf)
    *   rtw : W_n^k  = exp(sign*2*pi*i*k/n),  k in [0,n/2] (even r2c/c2r only) */
@@ -101,6 +116,39 @@ typedef struct pfk_pass {
   /* element counts of the src/dst buffers, for host-side bounds checks */
   int64_t src_elems, dst_elems;
 } pfk_pass;
+
+/* Longest computed line one workgroup transforms. */
+#define PFK_MAXLINE 4096
+
+/* Radix factorisation of a computed length: 4s first, then 2, then odd
+ * primes; the number of stages, or -1. */
+static inline int pfk_factorize(int64_t nf, int32_t *radix) {
+  int ns = 0;
+  int64_t m = nf;
+  while (m % 4 == 0 && ns < PFK_MAXSTAGES) { radix[ns++] = 4; m /= 4; }
+  while (m % 2 == 0 && ns < PFK_MAXSTAGES) { radix[ns++] = 2; m /= 2; }
+  for (int64_t q = 3; m > 1 && ns < PFK_MAXSTAGES; q += 2)
+    while (m % q == 0 && ns < PFK_MAXSTAGES) { radix[ns++] = (int32_t)q; m /= q; }
+  return m == 1 ? ns : -1;
+}
+
+/* Split of a long computed length nf = n1 * n2 with both factors at most
+ * PFK_MAXLINE: the most balanced one (smallest max/min) among those with a
+ * power-of-two n2 >= 64 (the second step then runs on the power-of-two
+ * register kernels), else among all; ties go to the larger n2.  Returns
+ * n1, or 0 if there is no split. */
+static inline int64_t pfk_long_split(int64_t nf) {
+  int64_t best[2] = {0, 0};
+  double ratio[2] = {0, 0};
+  for (int64_t n2 = PFK_MAXLINE; n2 >= 2; n2--) {
+    const int64_t n1 = nf / n2;
+    if (nf % n2 || n1 > PFK_MAXLINE || n1 < 2) continue;
+    const double r = n1 > n2 ? (double)n1 / (double)n2 : (double)n2 / (double)n1;
+    const int c = (n2 >= 64 && (n2 & (n2 - 1)) == 0) ? 0 : 1;
+    if (!best[c] || r < ratio[c]) { best[c] = n1; ratio[c] = r; }
+  }
+  return best[0] ? best[0] : best[1];
+}
 
 /* One task of a batched execution: one pass of one plan, with resolved
  * device pointers.  `dep` is the index of the task that must have finished
@@ -163,6 +211,10 @@ const char *pfk_last_error(void);
 
 /* Launch one pass on the library stream (asynchronous). */
 int  pfk_launch_pass(const pfk_pass *p, const void *src, void *dst);
+/* 0 if a kernel can run the (one-workgroup-per-line) pass: one of the
+ * specialised families covers it or the general kernel's line fits the
+ * device's LDS; else nonzero with the reason in pfk_last_error(). */
+int  pfk_pass_check(const pfk_pass *p);
 /* Kernel families with a kernel for the pass shape (bit mask):
  * PFK_HAS_POW2 register kernels for power-of-two lengths 64..4096,
  * PFK_HAS_SMALL one-line-per-lane kernels for short c2c lengths,

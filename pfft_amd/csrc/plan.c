@@ -64,6 +64,7 @@ struct pa_plan {
   int is_r2r;
   void *tw_bufs[4 * PA_MAXRNK];
   int ntw;
+  void *work[PA_MAXRNK];         /* long-line passes: lines * nf complex elements */
 };
 
 void pa_error(const char *fmt, ...) {
@@ -107,18 +108,6 @@ static int same_geo(const arrgeo *a, const arrgeo *b) {
   return 1;
 }
 
-/* small-prime factorisation of the computed FFT length into radices;
- * 4s first, then 2, then odd primes */
-static int factorize(INT nf, int32_t *radix) {
-  int ns = 0;
-  INT m = nf;
-  while (m % 4 == 0 && ns < PFK_MAXSTAGES) { radix[ns++] = 4; m /= 4; }
-  while (m % 2 == 0 && ns < PFK_MAXSTAGES) { radix[ns++] = 2; m /= 2; }
-  for (INT p = 3; m > 1 && ns < PFK_MAXSTAGES; p += 2)
-    while (m % p == 0 && ns < PFK_MAXSTAGES) { radix[ns++] = (int32_t)p; m /= p; }
-  if (m != 1) return -1;
-  return ns;
-}
 
 static const void *make_twiddles(pa_plan *p, INT N, INT count, int sign) {
   double *h = (double *)malloc(sizeof(double) * 2 * (size_t)(count > 0 ? count : 1));
@@ -137,6 +126,38 @@ static const void *make_twiddles(pa_plan *p, INT N, INT count, int sign) {
     free(f);
   }
   free(h);
+  p->tw_bufs[p->ntw++] = d;
+  return d;
+}
+
+/* Table of a long line nf = n1 * n2 (pfk.h): W_n1^m (m < n1), W_n2^m
+ * (m < n2), W_nf^l (l < 4096), W_nf^(4096 h) (h <= nf / 4096). */
+static const void *make_long_twiddles(pa_plan *p, INT nf, INT n1, int sign) {
+  const INT n2 = nf / n1, nh = nf / 4096 + 1;
+  const INT count = n1 + n2 + 4096 + nh;
+  double *h = (double *)malloc(sizeof(double) * 2 * (size_t)count);
+  if (!h) return NULL;
+  pa_twiddles((long)n1, (long)n1, sign, h);
+  pa_twiddles((long)n2, (long)n2, sign, h + 2 * n1);
+  pa_twiddles((long)nf, 4096, sign, h + 2 * (n1 + n2));
+  pa_twiddles_step((long)nf, (long)nh, 4096, sign, h + 2 * (n1 + n2 + 4096));
+  void *d = NULL;
+  int err;
+  if (p->prec == PFK_F64) {
+    const size_t bytes = sizeof(double) * 2 * (size_t)count;
+    err = pfk_malloc(&d, bytes) || pfk_memcpy_h2d(d, h, bytes);
+  } else {
+    float *f = (float *)malloc(sizeof(float) * 2 * (size_t)count);
+    err = !f;
+    if (f) {
+      for (INT i = 0; i < 2 * count; i++) f[i] = (float)h[i];
+      const size_t bytes = sizeof(float) * 2 * (size_t)count;
+      err = pfk_malloc(&d, bytes) || pfk_memcpy_h2d(d, f, bytes);
+      free(f);
+    }
+  }
+  free(h);
+  if (err) { pfk_free(d); return NULL; }
   p->tw_bufs[p->ntw++] = d;
   return d;
 }
@@ -226,6 +247,7 @@ static void free_plan_memory(pa_plan *p) {
   for (int i = 0; i < p->ntw; i++) pfk_free(p->tw_bufs[i]);
   p->ntw = 0;
   for (int i = 0; i < 2; i++) { pfk_free(p->scratch[i]); p->scratch[i] = NULL; }
+  for (int k = 0; k < PA_MAXRNK; k++) { pfk_free(p->work[k]); p->work[k] = NULL; }
 }
 
 /* Chooses every pass's launch variant (tune.c) on temporary device
@@ -498,11 +520,31 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     ps->omod.start = p->local_o_start[d];
     ps->omod.half = no[d] / 2;
     ps->omod.extra = 1;
+    if (kind != PFK_COPY && ps->nf > PFK_MAXLINE) {
+      /* long lines: two steps through a work buffer (c2c only) */
+      if (kind != PFK_C2C) {
+        pa_error("dimension %d: %s lines of computed length %lld exceed %d (supported for c2c "
+                 "dimensions only)", d, kind == PFK_R2R ? "r2r" : "real", (long long)ps->nf,
+                 PFK_MAXLINE);
+        goto fail;
+      }
+      ps->n1 = pfk_long_split(ps->nf);
+      if (!ps->n1) {
+        pa_error("dimension %d: length %lld has no split into two factors <= %d", d,
+                 (long long)ps->nf, PFK_MAXLINE);
+        goto fail;
+      }
+    }
     if (kind != PFK_COPY) {
-      ps->nstages = factorize(ps->nf, ps->radix);
+      ps->nstages = pfk_factorize(ps->nf, ps->radix);
       if (ps->nstages < 0) { pa_error("cannot factor length %lld", (long long)ps->nf); goto fail; }
-      ps->tw = make_twiddles(p, ps->nf, ps->nf, ps->sign);
-      p->tw_adj[k] = make_twiddles(p, ps->nf, ps->nf, -ps->sign);
+      if (ps->n1) {
+        ps->tw = make_long_twiddles(p, ps->nf, ps->n1, ps->sign);
+        p->tw_adj[k] = make_long_twiddles(p, ps->nf, ps->n1, -ps->sign);
+      } else {
+        ps->tw = make_twiddles(p, ps->nf, ps->nf, ps->sign);
+        p->tw_adj[k] = make_twiddles(p, ps->nf, ps->nf, -ps->sign);
+      }
       if (!ps->tw || !p->tw_adj[k]) goto fail;
       if ((kind == PFK_R2C || kind == PFK_C2R) && n[d] % 2 == 0) {
         ps->rtw = make_twiddles(p, n[d], n[d] / 2 + 1, sign);
@@ -512,6 +554,19 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     }
     if (fill_geometry(ps, &cur, &nxt, d)) goto fail;
     if (check_bounds(ps)) goto fail;
+    if (ps->n1) {
+      int64_t lines = ps->text;
+      for (int i = 0; i < ps->nb; i++) lines *= ps->bext[i];
+      const size_t wb = (size_t)(lines * ps->nf) * (prec == PFK_F64 ? 16 : 8);
+      if (pfk_malloc(&p->work[k], wb)) {
+        pa_error("device allocation of %zu bytes failed: %s", wb, pfk_last_error());
+        goto fail;
+      }
+      ps->work = p->work[k];
+    } else if (pfk_pass_check(ps)) {
+      pa_error("dimension %d: %s", d, pfk_last_error());
+      goto fail;
+    }
     /* buffers */
     p->src_role[k] = src_role;
     int dst_role;
@@ -709,9 +764,11 @@ void pa_plans_execute_batch(const pa_plan *const *plans, int count) {
     if (plans[i]->prec != plans[0]->prec || !pfk_is_device_ptr(plans[i]->in) ||
         !pfk_is_device_ptr(plans[i]->out))
       uniform = 0;
+    for (int k = 0; k < plans[i]->npass; k++)
+      if (plans[i]->pass[k].n1) uniform = 0;   /* long lines run in two launches */
   }
   if (!uniform) {
-    /* host arrays or mixed precisions: one plan after another */
+    /* host arrays, mixed precisions or long lines: one plan after another */
     for (int i = 0; i < count; i++) run(plans[i], plans[i]->in, plans[i]->out, 0);
     return;
   }
@@ -791,6 +848,7 @@ int pa_plan_npasses(const pa_plan *p) { return p ? p->npass : 0; }
 
 static const char *kernel_name(const pfk_pass *ps) {
   if (ps->kind == PFK_COPY) return "copy";
+  if (ps->n1) return "long-two-step";
   const int fam = pfk_pass_kernels(ps);
   const int lm = (ps->variant & 8) && ps->kind == PFK_C2C && ps->prec == PFK_F32;
   if (fam & PFK_HAS_MFMA) {

@@ -30,8 +30,13 @@ static void quarter_sincos(long r, long N, long double *c, long double *s) {
 }
 
 void pa_twiddles(long N, long count, int sign, double *out) {
-  for (long m = 0; m < count; m++) {
-    long mm = m % N;
+  pa_twiddles_step(N, count, 1, sign, out);
+}
+
+/* exp(sign*2*pi*i*step*j/N) for j in [0, count) */
+void pa_twiddles_step(long N, long count, long step, int sign, double *out) {
+  for (long j = 0; j < count; j++) {
+    long mm = (long)(((long long)step * j) % N);
     if (mm < 0) mm += N;
     const long long four_m = 4LL * mm;
     const long q = (long)(four_m / N);
@@ -44,7 +49,7 @@ void pa_twiddles(long N, long count, int sign, double *out) {
       case 2: cc = -c; ss = -s; break;
       default: cc = s; ss = -c; break;
     }
-    out[2 * m] = (double)cc;
-    out[2 * m + 1] = (double)(sign < 0 ? -ss : ss);
+    out[2 * j] = (double)cc;
+    out[2 * j + 1] = (double)(sign < 0 ? -ss : ss);
   }
 }
