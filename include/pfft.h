@@ -101,6 +101,7 @@ typedef float pfftf_complex[2];
   typedef struct P##plan_s *P##plan;                                                \
   typedef struct P##gcplan_s *P##gcplan;                                            \
   typedef struct P##gctimer_s *P##gctimer;                                          \
+  typedef struct P##timer_s *P##timer;                                              \
   /* setup: api/pfft.h:77-78 (init, cleanup), 79-80 (threads) */                   \
   void P##init(void);                                                               \
   void P##cleanup(void);                                                            \
@@ -299,6 +300,25 @@ typedef float pfftf_complex[2];
   int P##create_procmesh_1d(MPI_Comm comm, int np0, MPI_Comm *comm_cart_1d);        \
   int P##create_procmesh_2d(MPI_Comm comm, int np0, int np1,                        \
       MPI_Comm *comm_cart_2d);                                                      \
+  /* plan timers: api/pfft.h:386-412.  Each execute adds its wall time to     \
+   * `whole` and each pass's device time to the trafo slot of the reference  \
+   * step that transforms that dimension; remaps and twiddles are fused into \
+   * the passes and read 0.  convert_timer2vec returns malloc'd memory. */     \
+  void P##reset_timer(P##plan ths);                                                 \
+  P##timer P##get_timer(const P##plan ths);                                         \
+  void P##print_average_timer(const P##plan ths, MPI_Comm comm);                    \
+  void P##print_average_timer_adv(const P##plan ths, MPI_Comm comm);                \
+  void P##write_average_timer(const P##plan ths, const char *name,                  \
+      MPI_Comm comm);                                                               \
+  void P##write_average_timer_adv(const P##plan ths, const char *name,              \
+      MPI_Comm comm);                                                               \
+  P##timer P##copy_timer(const P##timer orig);                                      \
+  void P##average_timer(P##timer ths);                                              \
+  P##timer P##add_timers(const P##timer sum1, const P##timer sum2);                 \
+  P##timer P##reduce_max_timer(const P##timer ths, MPI_Comm comm);                  \
+  double *P##convert_timer2vec(const P##timer ths);                                 \
+  P##timer P##convert_vec2timer(const double *times);                               \
+  void P##destroy_timer(P##timer ths);                                              \
   /* ghost cells: api/pfft.h:433-501.  Three-dimensional arrays; on one         \
    * process the ghost cells are periodic images of the local block.  The       \
    * array must hold local_size_*gc reals / complex values. */                  \

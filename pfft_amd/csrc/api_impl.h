@@ -603,3 +603,171 @@ void PA(write_average_gctimer)(const PA(gcplan) ths, const char *name, MPI_Comm 
 void PA(write_average_gctimer_adv)(const PA(gcplan) ths, const char *name, MPI_Comm comm) {
   PA(gc_write)(ths, name, comm, 1);
 }
+
+/* ---------------- plan timers: kernel/timer.c, api/pfft.h:386-412 ---------------- */
+static PA(timer) PA(timer_dup)(const pa_timer *src) {
+  pa_timer *t = (pa_timer *)malloc(sizeof(pa_timer));
+  if (t && src) *t = *src;
+  return (PA(timer))t;
+}
+void PA(reset_timer)(PA(plan) ths) {
+  if (!ths) return;
+  pa_timer *t = pa_plan_timer((const pa_plan *)ths);
+  pa_timer *z = pa_timer_new(t->rnk_pm);
+  if (z) { *t = *z; free(z); }
+}
+PA(timer) PA(get_timer)(const PA(plan) ths) {
+  return ths ? PA(timer_dup)(pa_plan_timer((const pa_plan *)ths)) : NULL;
+}
+PA(timer) PA(copy_timer)(const PA(timer) orig) {
+  return orig ? PA(timer_dup)((const pa_timer *)orig) : NULL;
+}
+void PA(average_timer)(PA(timer) ths) {
+  pa_timer *t = (pa_timer *)ths;
+  if (!t || t->iter <= 0) return;
+  t->whole /= t->iter;
+  for (int k = 0; k < t->rnk_trafo; k++) t->trafo[k] /= t->iter;
+  for (int k = 0; k < t->rnk_remap; k++) t->remap[k] /= t->iter;
+  for (int k = 0; k < 2; k++) t->remap_3dto2d[k] /= t->iter;
+  t->itwiddle /= t->iter;
+  t->otwiddle /= t->iter;
+  t->iter = 1;
+}
+PA(timer) PA(add_timers)(const PA(timer) sum1, const PA(timer) sum2) {
+  pa_timer *r = (pa_timer *)PA(copy_timer)(sum1);
+  const pa_timer *b = (const pa_timer *)sum2;
+  if (!r || !b) return (PA(timer))r;
+  r->iter += b->iter;
+  r->whole += b->whole;
+  for (int k = 0; k < r->rnk_trafo; k++) r->trafo[k] += b->trafo[k];
+  for (int k = 0; k < r->rnk_remap; k++) r->remap[k] += b->remap[k];
+  for (int k = 0; k < 2; k++) r->remap_3dto2d[k] += b->remap_3dto2d[k];
+  r->itwiddle += b->itwiddle;
+  r->otwiddle += b->otwiddle;
+  return (PA(timer))r;
+}
+/* one process: the maximum over ranks is the timer itself */
+PA(timer) PA(reduce_max_timer)(const PA(timer) ths, MPI_Comm comm) {
+  (void)comm;
+  return PA(copy_timer)(ths);
+}
+/* [rnk_pm, rnk_trafo, rnk_remap, iter, whole, trafo..., remap...,
+ *  remap_3dto2d[2], itwiddle, otwiddle] (kernel/timer.c:297-340); the
+ * caller frees the vector */
+double *PA(convert_timer2vec)(const PA(timer) ths) {
+  const pa_timer *t = (const pa_timer *)ths;
+  double *v = (double *)malloc(sizeof(double) * (size_t)(t->rnk_trafo + t->rnk_remap + 9));
+  if (!v) return NULL;
+  int m = 0;
+  v[m++] = t->rnk_pm;
+  v[m++] = t->rnk_trafo;
+  v[m++] = t->rnk_remap;
+  v[m++] = t->iter;
+  v[m++] = t->whole;
+  for (int k = 0; k < t->rnk_trafo; k++) v[m++] = t->trafo[k];
+  for (int k = 0; k < t->rnk_remap; k++) v[m++] = t->remap[k];
+  v[m++] = t->remap_3dto2d[0];
+  v[m++] = t->remap_3dto2d[1];
+  v[m++] = t->itwiddle;
+  v[m++] = t->otwiddle;
+  return v;
+}
+PA(timer) PA(convert_vec2timer)(const double *times) {
+  pa_timer *t = pa_timer_new((int)times[0]);
+  if (!t || t->rnk_pm < 0 || t->rnk_pm > PA_MAXRNK) { free(t); return NULL; }
+  int m = 3;
+  t->iter = (int)times[m++];
+  t->whole = times[m++];
+  for (int k = 0; k < t->rnk_trafo; k++) t->trafo[k] = times[m++];
+  for (int k = 0; k < t->rnk_remap; k++) t->remap[k] = times[m++];
+  t->remap_3dto2d[0] = times[m++];
+  t->remap_3dto2d[1] = times[m++];
+  t->itwiddle = times[m++];
+  t->otwiddle = times[m++];
+  return (PA(timer))t;
+}
+void PA(destroy_timer)(PA(timer) ths) { free(ths); }
+
+/* output format of kernel/timer.c:114-220, 402-455 (Matlab assignments;
+ * index 1 = log2(1 process) + 1) */
+static void PA(timer_print)(const PA(plan) ths, MPI_Comm comm, FILE *f, int header, int adv) {
+  int rnk, sign;
+  const INT *n, *ni, *no;
+  unsigned fl;
+  pa_plan_info((const pa_plan *)ths, &rnk, &n, &ni, &no, &fl, &sign);
+  const pa_timer *t = pa_plan_timer((const pa_plan *)ths);
+  if (header) {
+    PA(fprintf)(comm, f, "\n%% n - FFT size\n");
+    PA(fprintf)(comm, f, "%% np - process grid\n");
+    PA(fprintf)(comm, f, "%% procs - number of processes\n");
+    PA(fprintf)(comm, f, "%% pfft - PFFT runtime\n");
+    PA(fprintf)(comm, f, "%% index(i) = log(procs(i)) + 1\n");
+  }
+  const char *effort = (fl & PFFT_ESTIMATE) ? "ESTIMATE" : (fl & PFFT_PATIENT) ? "PATIENT"
+                       : (fl & PFFT_EXHAUSTIVE) ? "EXHAUSTIVE" : "MEASURE";
+  PA(fprintf)(comm, f, "%% pfft_flags == PFFT_%s", effort);
+  if (fl & PFFT_TRANSPOSED_IN) PA(fprintf)(comm, f, " | PFFT_TRANSPOSED_IN");
+  if (fl & PFFT_TRANSPOSED_OUT) PA(fprintf)(comm, f, " | PFFT_TRANSPOSED_OUT");
+  if (fl & PFFT_SHIFTED_IN) PA(fprintf)(comm, f, " | PFFT_SHIFTED_IN");
+  if (fl & PFFT_SHIFTED_OUT) PA(fprintf)(comm, f, " | PFFT_SHIFTED_OUT");
+  PA(fprintf)(comm, f, (fl & PFFT_TUNE) ? " | PFFT_TUNE" : " | PFFT_NO_TUNE");
+  if (fl & PFFT_PRESERVE_INPUT) PA(fprintf)(comm, f, " | PFFT_PRESERVE_INPUT");
+  if (fl & PFFT_DESTROY_INPUT) PA(fprintf)(comm, f, " | PFFT_DESTROY_INPUT");
+  if (fl & PFFT_BUFFERED_INPLACE) PA(fprintf)(comm, f, " | PFFT_BUFFERED_INPLACE");
+  PA(fprintf)(comm, f, "\n%% fftw_flags == FFTW_%s\n", effort);
+  PA(fprintf)(comm, f, "index(%d) = %d;  procs(%d) = %d;  ", 1, 1, 1, 1);
+  PA(fprintf)(comm, f, "np_pfft(%d, 1:%d) = [", 1, t->rnk_pm);
+  for (int k = 0; k < t->rnk_pm; k++) PA(fprintf)(comm, f, "%d ", 1);
+  PA(fprintf)(comm, f, "];\n");
+  const INT *vs[3] = {n, ni, no};
+  const char *nm[3] = {"n_pfft", "ni_pfft", "no_pfft"};
+  for (int v = 0; v < 3; v++) {
+    PA(fprintf)(comm, f, "%s(%d, 1:%d) = [", nm[v], 1, rnk);
+    for (int k = 0; k < rnk; k++) PA(fprintf)(comm, f, "%td ", vs[v][k]);
+    PA(fprintf)(comm, f, v < 2 ? "];  " : "];\n");
+  }
+  const char *pre = (sign == PFFT_FORWARD) ? "pfft_forw" : "pfft_back";
+  pa_timer m = *t;
+  PA(average_timer)((PA(timer))&m);
+  PA(fprintf)(comm, f, "%s_iter(%d)    = %d;  ", pre, 1, t->iter);
+  PA(fprintf)(comm, f, "%s(%d)   = %.3e;\n", pre, 1, m.whole);
+  if (!adv) return;
+  int k = 0, l = 0;
+  PA(fprintf)(comm, f, "%s_itwiddle(%d)   = %.3e;\n", pre, 1, m.itwiddle);
+  PA(fprintf)(comm, f, "%s_remap_3dto2d(%d, 2)   = %.3e;\n", pre, 1, m.remap_3dto2d[0]);
+  for (int h = 0; h < 2; h++) {   /* the T_OUT steps, then the T_IN steps */
+    for (int q = 0; q < m.rnk_pm; q++, k++, l++) {
+      PA(fprintf)(comm, f, "%s_trafo%d(%d, 2)   = %.3e;  ", pre, k + 1, 1, m.trafo[k]);
+      PA(fprintf)(comm, f, "%s_remap%d(%d, 1) = %.3e;\n", pre, l + 1, 1, m.remap[l]);
+    }
+    PA(fprintf)(comm, f, "%s_trafo%d(%d, 2)   = %.3e;\n", pre, k + 1, 1, m.trafo[k]);
+    k++;
+  }
+  PA(fprintf)(comm, f, "%s_remap_2dto3d(%d, 2)   = %.3e;\n", pre, 1, m.remap_3dto2d[1]);
+  PA(fprintf)(comm, f, "%s_otwiddle(%d)   = %.3e;\n", pre, 1, m.otwiddle);
+}
+void PA(print_average_timer)(const PA(plan) ths, MPI_Comm comm) {
+  if (ths) PA(timer_print)(ths, comm, stdout, 1, 0);
+}
+void PA(print_average_timer_adv)(const PA(plan) ths, MPI_Comm comm) {
+  if (ths) PA(timer_print)(ths, comm, stdout, 1, 1);
+}
+static void PA(timer_write)(const PA(plan) ths, const char *name, MPI_Comm comm, int adv) {
+  if (!ths || !name) return;
+  FILE *probe = fopen(name, "r");
+  const int newfile = (probe == NULL);
+  if (probe) fclose(probe);
+  FILE *f = fopen(name, "a+");
+  if (!f) {
+    PA(fprintf)(comm, stderr, "Error: Cannot open file %s.\n", name);
+    return;
+  }
+  PA(timer_print)(ths, comm, f, newfile, adv);
+  fclose(f);
+}
+void PA(write_average_timer)(const PA(plan) ths, const char *name, MPI_Comm comm) {
+  PA(timer_write)(ths, name, comm, 0);
+}
+void PA(write_average_timer_adv)(const PA(plan) ths, const char *name, MPI_Comm comm) {
+  PA(timer_write)(ths, name, comm, 1);
+}

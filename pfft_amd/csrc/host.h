@@ -67,6 +67,21 @@ int pa_plan_time_passes(const pa_plan *p, void *in, void *out, int iters, double
 int pa_plan_npasses(const pa_plan *p);
 int pa_plan_describe(const pa_plan *p, char *buf, size_t len);
 
+/* execution timer: the fields of the reference's timer (kernel/ipfft.h:
+ * 208-219, kernel/timer.c:358-373), in seconds */
+typedef struct pa_timer {
+  int rnk_pm, rnk_trafo, rnk_remap, iter;
+  double whole;
+  double trafo[2 * PA_MAXRNK + 2];
+  double remap[2 * PA_MAXRNK];
+  double remap_3dto2d[2];
+  double itwiddle, otwiddle;
+} pa_timer;
+pa_timer *pa_timer_new(int rnk_pm);
+pa_timer *pa_plan_timer(const pa_plan *p);
+void pa_plan_info(const pa_plan *p, int *rnk_n, const INT **n, const INT **ni, const INT **no,
+                  unsigned *pfft_flags, int *sign);
+
 /* ---- autotuner and wisdom (tune.c) ---- */
 int pa_tune_lookup(pfk_pass *ps, unsigned effort);
 int pa_tune_measure(pfk_pass *ps, const void *src, void *dst, unsigned effort);

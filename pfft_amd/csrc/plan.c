@@ -27,6 +27,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include "host.h"
 
 enum { ROLE_IN = 0, ROLE_OUT = 1, ROLE_S0 = 2, ROLE_S1 = 3 };
@@ -65,6 +66,11 @@ struct pa_plan {
   void *tw_bufs[4 * PA_MAXRNK];
   int ntw;
   void *work[PA_MAXRNK];         /* long-line passes: lines * nf complex elements */
+  /* execution timer (kernel/timer.c fields): pass k adds its device time
+   * to trafo[tslot[k]]; events tev[0..npass] bracket the passes */
+  pa_timer *timer;
+  int tslot[PA_MAXRNK];
+  void *tev[PA_MAXRNK + 1];
 };
 
 void pa_error(const char *fmt, ...) {
@@ -243,7 +249,17 @@ static int64_t geo_elems(const arrgeo *g) {
   return geo_strides(g, s);
 }
 
+static double wall_seconds(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 static void free_plan_memory(pa_plan *p) {
+  for (int k = 0; k <= PA_MAXRNK; k++)
+    if (p->tev[k]) { pfk_event_destroy(p->tev[k]); p->tev[k] = NULL; }
+  free(p->timer);
+  p->timer = NULL;
   for (int i = 0; i < p->ntw; i++) pfk_free(p->tw_bufs[i]);
   p->ntw = 0;
   for (int i = 0; i < 2; i++) { pfk_free(p->scratch[i]); p->scratch[i] = NULL; }
@@ -597,6 +613,24 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
       }
     }
   }
+  /* timer (kernel/partrafo.c:895, kernel/timer.c:358-373).  Each pass does
+   * one dimension's FFT with the remaps fused into its strides; its time
+   * goes to the trafo slot of the reference step that transforms that
+   * dimension: the T_OUT pipeline does dims r..R-1 in trafo[0] and dim d < r
+   * in trafo[r-d]; the T_IN pipeline (TRANSPOSED_IN plans and c2r) dim
+   * d < r in trafo[r+1+d] and dims r..R-1 in trafo[2r+1]
+   * (api/api-basic.c:1071-1106, 1288-1311). */
+  p->timer = pa_timer_new(r);
+  if (!p->timer) goto fail;
+  for (int k = 0; k < R; k++) {
+    const int d = ord[k];
+    p->tslot[k] = first0 ? (d >= r ? 2 * r + 1 : r + 1 + d) : (d >= r ? 0 : r - d);
+  }
+  for (int k = 0; k <= R; k++)
+    if (pfk_event_create(&p->tev[k])) {
+      pa_error("event creation failed: %s", pfk_last_error());
+      goto fail;
+    }
   tune_plan(p);
   return p;
 fail:
@@ -620,6 +654,7 @@ void *pa_plan_out(const pa_plan *p) { return p->out; }
 /* Runs the passes; `adjoint` runs them in reverse with swapped roles. */
 static void run(const pa_plan *p, void *in, void *out, int adjoint) {
   if (!p) return;
+  const double t0 = wall_seconds();
   if (adjoint && p->is_r2r) {
     pa_error("the adjoint pipeline is not available for r2r plans");
     return;
@@ -655,7 +690,7 @@ static void run(const pa_plan *p, void *in, void *out, int adjoint) {
     }
   }
   void *bufs[4] = {din, dout, p->scratch[0], p->scratch[1]};
-  int err = 0;
+  int err = pfk_event_record(p->tev[0]);
   for (int i = 0; i < p->npass && !err; i++) {
     const int k = adjoint ? p->npass - 1 - i : i;
     const pfk_pass *fw = &p->pass[k];
@@ -678,17 +713,49 @@ static void run(const pa_plan *p, void *in, void *out, int adjoint) {
       static const int swap_io[4] = {ROLE_OUT, ROLE_IN, ROLE_S0, ROLE_S1};
       err = pfk_launch_pass(&a, bufs[swap_io[p->dst_role[k]]], bufs[swap_io[p->src_role[k]]]);
     }
+    if (!err) err = pfk_event_record(p->tev[i + 1]);
     if (err) pa_error("pass %d launch failed: %s", k, pfk_last_error());
   }
-  if (!err && pfk_sync()) pa_error("execution failed: %s", pfk_last_error());
+  if (!err && pfk_sync()) { pa_error("execution failed: %s", pfk_last_error()); err = 1; }
+  if (!err) {
+    for (int i = 0; i < p->npass; i++) {
+      float ms = 0;
+      const int k = adjoint ? p->npass - 1 - i : i;
+      if (!pfk_event_elapsed_ms(p->tev[i], p->tev[i + 1], &ms))
+        p->timer->trafo[p->tslot[k]] += 1e-3 * ms;
+    }
+  }
   if (host_out && !err) {
     if (pfk_memcpy_d2h(out, dout, out_bytes)) pa_error("copy-out failed: %s", pfk_last_error());
   }
   pfk_free(stage_in);
   pfk_free(stage_out);
+  p->timer->whole += wall_seconds() - t0;
+  p->timer->iter++;
 }
 
 void pa_plan_execute(const pa_plan *p, void *in, void *out) { run(p, in, out, 0); }
+
+pa_timer *pa_plan_timer(const pa_plan *p) { return p->timer; }
+
+void pa_plan_info(const pa_plan *p, int *rnk_n, const INT **n, const INT **ni, const INT **no,
+                  unsigned *pfft_flags, int *sign) {
+  *rnk_n = p->rnk_n;
+  *n = p->n;
+  *ni = p->ni;
+  *no = p->no;
+  *pfft_flags = p->pfft_flags;
+  *sign = p->sign;
+}
+
+pa_timer *pa_timer_new(int rnk_pm) {
+  pa_timer *t = (pa_timer *)calloc(1, sizeof(pa_timer));
+  if (!t) return NULL;
+  t->rnk_pm = rnk_pm;
+  t->rnk_trafo = 2 * rnk_pm + 2;
+  t->rnk_remap = 2 * rnk_pm;
+  return t;
+}
 void pa_plan_execute_adjoint(const pa_plan *p, void *in, void *out) { run(p, in, out, 1); }
 
 /* ---- batched execution ----------------------------------------------
