@@ -83,6 +83,15 @@ typedef float pfftf_complex[2];
 /* Ghost-cell planner flags: same bit values as the reference
  * (api/pfft.h:563-572).  PFFT_GC_SENDRECV and PFFT_GC_RMA choose an MPI
  * algorithm in the reference and are accepted and ignored here. */
+/* value types of pfft_get_args (api/pfft.h:555-561) */
+#define PFFT_INT                  (1U)
+#define PFFT_PTRDIFF_T            (2U)
+#define PFFT_FLOAT                (3U)
+#define PFFT_DOUBLE               (4U)
+#define PFFT_LDOUBLE              (5U)
+#define PFFT_UNSIGNED             (6U)
+#define PFFT_SWITCH               (7U)
+
 #define PFFT_GC_TRANSPOSED_NONE   (0U)
 #define PFFT_GC_TRANSPOSED        (1U<< 0)
 #define PFFT_GC_SENDRECV          (1U<< 1)
@@ -300,6 +309,20 @@ typedef float pfftf_complex[2];
   int P##create_procmesh_1d(MPI_Comm comm, int np0, MPI_Comm *comm_cart_1d);        \
   int P##create_procmesh_2d(MPI_Comm comm, int np0, int np1,                        \
       MPI_Comm *comm_cart_2d);                                                      \
+  /* array printing and command-line options: api/pfft.h:361-383 (one rank;  \
+   * device arrays are copied to the host for printing) */                     \
+  void P##apr_complex_3d(const C *data, const INT *local_n,                        \
+      const INT *local_start, const char *name, MPI_Comm comm);                     \
+  void P##apr_complex_permuted_3d(const C *data, const INT *local_n,               \
+      const INT *local_start, int perm0, int perm1, int perm2, const char *name,    \
+      MPI_Comm comm);                                                               \
+  void P##apr_real_3d(const R *data, const INT *local_n,                           \
+      const INT *local_start, const char *name, MPI_Comm comm);                     \
+  void P##apr_real_permuted_3d(const R *data, const INT *local_n,                  \
+      const INT *local_start, int perm0, int perm1, int perm2, const char *name,    \
+      MPI_Comm comm);                                                               \
+  void P##get_args(int argc, char **argv, const char *name, int neededArgs,        \
+      unsigned type, void *parameter);                                              \
   /* plan timers: api/pfft.h:386-412.  Each execute adds its wall time to     \
    * `whole` and each pass's device time to the trafo slot of the reference  \
    * step that transforms that dimension; remaps and twiddles are fused into \

@@ -771,3 +771,101 @@ void PA(write_average_timer)(const PA(plan) ths, const char *name, MPI_Comm comm
 void PA(write_average_timer_adv)(const PA(plan) ths, const char *name, MPI_Comm comm) {
   PA(timer_write)(ths, name, comm, 1);
 }
+
+/* ---------------- array printing: api/api-basic.c:584-710 ---------------- */
+/* One rank; device arrays are copied to the host first.  Format of the
+ * reference: a header line per k0 slice, then rows of "%.2e" values. */
+static void PA(apr)(const R *data, const INT *ln, const INT *ls, const char *name, int cplx) {
+  const INT cnt = ln[0] * ln[1] * ln[2];
+  if (cnt < 1 || !data) return;
+  const size_t bytes = sizeof(R) * (size_t)cnt * (cplx ? 2 : 1);
+  const R *h = data;
+  R *copy = NULL;
+  if (pfk_is_device_ptr(data)) {
+    copy = (R *)malloc(bytes);
+    if (!copy || pfk_memcpy_d2h(copy, data, bytes)) { free(copy); return; }
+    h = copy;
+  }
+  fflush(stdout);
+  printf("Rank %d:\n", 0);
+  INT l = 0;
+  for (INT k0 = 0; k0 < ln[0]; k0++) {
+    printf("%s(%td,%td:%td,%td:%td):\n", name, ls[0] + k0, ls[1], ls[1] + ln[1] - 1, ls[2],
+           ls[2] + ln[2] - 1);
+    for (INT k1 = 0; k1 < ln[1]; k1++) {
+      printf("  ");
+      for (INT k2 = 0; k2 < ln[2]; k2++, l++) {
+        if (cplx) printf("  %.2e + %.2ei,", (double)h[2 * l], (double)h[2 * l + 1]);
+        else printf("  %.2e,", (double)h[l]);
+      }
+      printf("\n");
+    }
+  }
+  printf("\n");
+  fflush(stdout);
+  free(copy);
+}
+void PA(apr_complex_3d)(const CPLX *data, const INT *local_n, const INT *local_start,
+                        const char *name, MPI_Comm comm) {
+  (void)comm;
+  PA(apr)((const R *)data, local_n, local_start, name, 1);
+}
+void PA(apr_real_3d)(const R *data, const INT *local_n, const INT *local_start,
+                     const char *name, MPI_Comm comm) {
+  (void)comm;
+  PA(apr)(data, local_n, local_start, name, 0);
+}
+void PA(apr_complex_permuted_3d)(const CPLX *data, const INT *local_n, const INT *local_start,
+                                 int perm0, int perm1, int perm2, const char *name,
+                                 MPI_Comm comm) {
+  const int pm[3] = {perm0, perm1, perm2};
+  INT n[3], s[3];
+  for (int t = 0; t < 3; t++) { n[t] = local_n[pm[t]]; s[t] = local_start[pm[t]]; }
+  PA(apr_complex_3d)(data, n, s, name, comm);
+}
+void PA(apr_real_permuted_3d)(const R *data, const INT *local_n, const INT *local_start,
+                              int perm0, int perm1, int perm2, const char *name,
+                              MPI_Comm comm) {
+  const int pm[3] = {perm0, perm1, perm2};
+  INT n[3], s[3];
+  for (int t = 0; t < 3; t++) { n[t] = local_n[pm[t]]; s[t] = local_start[pm[t]]; }
+  PA(apr_real_3d)(data, n, s, name, comm);
+}
+
+/* ---------------- command-line options: util/getargs.c:41-130 ---------------- */
+/* `name` followed by neededArgs values of `type` (PFFT_INT ... PFFT_UNSIGNED)
+ * is read into parameter[0..neededArgs); PFFT_SWITCH sets *(int*)parameter
+ * to 1 when `name` is present.  Values are counted up to the next argument
+ * that starts with '-'.  An unsupported type reports an error and reads
+ * nothing. */
+void PA(get_args)(int argc, char **argv, const char *name, int neededArgs, unsigned type,
+                  void *parameter) {
+  for (int it = 0; it < argc; it++) {
+    if (strcmp(argv[it], name)) continue;
+    if (type == PFFT_SWITCH) { *(int *)parameter = 1; return; }
+    int given = 0;
+    while (it + given + 1 < argc && argv[it + given + 1] && argv[it + given + 1][0] != '-')
+      given++;
+    if (given < neededArgs) {
+      PA(printf)(MPI_COMM_WORLD, "!!! Warning: Not enough command line arguments for %s !!!\n",
+                 name);
+      return;
+    }
+    for (int t = 0; t < neededArgs; t++) {
+      const char *a = argv[it + 1 + t];
+      switch (type) {
+        case PFFT_INT: ((int *)parameter)[t] = (int)strtol(a, NULL, 0); break;
+        case PFFT_PTRDIFF_T: ((ptrdiff_t *)parameter)[t] = (ptrdiff_t)strtol(a, NULL, 0); break;
+        case PFFT_FLOAT: ((float *)parameter)[t] = strtof(a, NULL); break;
+        case PFFT_DOUBLE: ((double *)parameter)[t] = strtod(a, NULL); break;
+        case PFFT_LDOUBLE: ((long double *)parameter)[t] = strtold(a, NULL); break;
+        case PFFT_UNSIGNED: ((unsigned *)parameter)[t] = (unsigned)strtoul(a, NULL, 0); break;
+        default:
+          PA(fprintf)(MPI_COMM_WORLD, stderr,
+                      "!!! Error: PFFT_DATATYPE of %s not supported. !!!\n", name);
+          return;
+      }
+    }
+    return;
+  }
+}

@@ -68,3 +68,51 @@ int main(void) {
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 0, (r.stdout, r.stderr)
     assert r.stdout.split() == ["24273", "29", "31", "0"]
+
+
+def test_get_args_and_array_printing(tmp_path):
+    """pfft_get_args (util/getargs.c) and pfft_apr_real_3d /
+    pfft_apr_complex_permuted_3d (api/api-basic.c:584-710) from a C
+    program, on host arrays."""
+    src = r'''
+#include <stdio.h>
+#include "pfft.h"
+int main(int argc, char **argv) {
+  ptrdiff_t n[3] = {2, 2, 2};
+  double tol = 0;
+  int sw = 0, miss = 7;
+  pfft_get_args(argc, argv, "-pfft_n", 3, PFFT_PTRDIFF_T, n);
+  pfft_get_args(argc, argv, "-tol", 1, PFFT_DOUBLE, &tol);
+  pfft_get_args(argc, argv, "-verbose", 0, PFFT_SWITCH, &sw);
+  pfft_get_args(argc, argv, "-absent", 1, PFFT_INT, &miss);
+  printf("%td %td %td %.3f %d %d\n", n[0], n[1], n[2], tol, sw, miss);
+  double r[4] = {1, 2, 3, 4};
+  ptrdiff_t ln[3] = {1, 2, 2}, ls[3] = {0, 1, 0};
+  pfft_apr_real_3d(r, ln, ls, "r", MPI_COMM_WORLD);
+  double c[4] = {1, -1, 2, -2};
+  ptrdiff_t lc[3] = {2, 1, 1}, lsc[3] = {3, 0, 0};
+  pfft_apr_complex_permuted_3d((const pfft_complex *)c, lc, lsc, 1, 2, 0, "c", MPI_COMM_WORLD);
+  return 0;
+}
+'''
+    c = str(tmp_path / "args_probe.c")
+    exe = str(tmp_path / "args_probe")
+    with open(c, "w") as f:
+        f.write(src)
+    libdir = os.path.dirname(P.LIB_PATH)
+    r = subprocess.run(["gcc", "-std=c99", "-I" + os.path.join(ROOT, "include"), c, "-o", exe,
+                        "-L" + libdir, "-lpfft", "-Wl,-rpath," + libdir],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe, "-pfft_n", "8", "16", "32", "-tol", "0.25", "-verbose"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.split("\n")
+    assert lines[0] == "8 16 32 0.250 1 7"
+    assert lines[1] == "Rank 0:"
+    assert lines[2] == "r(0,1:2,0:1):"
+    assert lines[3].split() == ["1.00e+00,", "2.00e+00,"]
+    assert lines[4].split() == ["3.00e+00,", "4.00e+00,"]
+    # permuted: sizes (1, 1, 2), starts (0, 0, 3)
+    assert "c(0,0:0,3:4):" in r.stdout
+    assert "1.00e+00 + -1.00e+00i," in r.stdout and "2.00e+00 + -2.00e+00i," in r.stdout
