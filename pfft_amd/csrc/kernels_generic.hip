@@ -131,7 +131,7 @@ int launch_generic(const pfk_pass *p, const void *src, void *dst) {
 // power-of-two register kernels (twiddled-store instantiation) unless the
 // line is embedded or modulated; step B on any kernel unless it truncates
 // or modulates.
-static int launch_long(const pfk_pass *p, const void *src, void *dst) {
+static int long_c2c(const pfk_pass *p, const void *src, void *dst) {
   const int64_t N = p->nf, N1 = p->n1, N2 = N / N1;
   if (p->kind != PFK_C2C || N1 < 2 || N1 * N2 != N || N1 > PFK_MAXLINE || N2 > PFK_MAXLINE ||
       !p->work) {
@@ -198,6 +198,151 @@ static int launch_long(const pfk_pass *p, const void *src, void *dst) {
   if (a.nstages < 0 || b.nstages < 0) { set_error_msg("pfk: long-line factors"); return 1; }
   if (pfk_launch_pass(&a, src, p->work)) return 1;
   return pfk_launch_pass(&b, p->work, dst);
+}
+
+// ---- long real lines (n even, M = n/2 > PFK_MAXLINE) ----
+// Line index l (tile index innermost, then the batch indices) -> offset.
+__device__ __forceinline__ int64_t line_offset(const pfk_pass &P, int64_t l, bool out) {
+  const int64_t t = l % P.text;
+  l /= P.text;
+  int64_t o = t * (out ? P.tsout : P.tsin);
+  for (int i = 0; i < P.nb; ++i) {
+    const int64_t d = l % P.bext[i];
+    l /= P.bext[i];
+    o += d * (out ? P.bsout[i] : P.bsin[i]);
+  }
+  return o;
+}
+
+// r2c: Z = DFT_M(z), z_j = x_2j + i x_2j+1, lines of M in `Z`.  Output k
+// (kept range [zout, zout+nout) of 0..M) is X_k = E_k + W_n^k O_k with
+// E_k = (Z_k + conj Z_(M-k))/2, O_k = (Z_k - conj Z_(M-k))/(2i); times 2
+// off DC/Nyquist for the adjoint weights (hw 2) and the output modulation.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_r2c_long_post(const pfk_pass P, const int64_t total, const cpx<T> *__restrict__ Z,
+                cpx<T> *__restrict__ dst) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= total) return;
+  const int64_t M = P.nf, k = g % P.nout, l = g / P.nout;
+  const int64_t kk = P.zout + k;
+  const cpx<T> *z = Z + l * M;
+  const cpx<T> a = z[kk % M], b0 = z[(M - kk) % M];
+  const cpx<T> b{b0.x, -b0.y};
+  const cpx<T> e{(a.x + b.x) * (T)0.5, (a.y + b.y) * (T)0.5};
+  const cpx<T> o{(a.y - b.y) * (T)0.5, -(a.x - b.x) * (T)0.5};   // (a - b) / (2i)
+  const cpx<T> w = reinterpret_cast<const cpx<T> *>(P.rtw)[kk];
+  cpx<T> x = cmul(w, o);
+  x.x += e.x;
+  x.y += e.y;
+  T f = (T)mod_sign(P.omod, k);
+  if (P.hw == 2 && kk > 0 && kk < M) f *= (T)2;
+  dst[line_offset(P, l, true) + k * P.sout] = cpx<T>{x.x * f, x.y * f};
+}
+
+// c2r: Z'_k = (X_k + conj X_(M-k)) + i W_n^k (X_k - conj X_(M-k)), k < M,
+// from the half spectrum (kept input range [zin, zin+nin) of 0..M, input
+// modulation, x1/2 off DC/Nyquist for the adjoint weights (hw 1), imaginary
+// parts of DC and Nyquist ignored); then x_2j + i x_2j+1 = DFT_M(Z')_j.
+template <typename T>
+__device__ __forceinline__ cpx<T> c2r_in(const pfk_pass &P, const cpx<T> *s, int64_t k) {
+  const int64_t j = k - P.zin;
+  if (j < 0 || j >= P.nin) return cpx<T>{0, 0};
+  cpx<T> v = s[j * P.sin];
+  T f = (T)mod_sign(P.imod, j);
+  if (P.hw == 1 && k > 0 && k < P.nf) f *= (T)0.5;
+  if (k == 0 || k == P.nf) v.y = 0;
+  return cpx<T>{v.x * f, v.y * f};
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_c2r_long_pre(const pfk_pass P, const int64_t total, const cpx<T> *__restrict__ src,
+               cpx<T> *__restrict__ Z) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= total) return;
+  const int64_t M = P.nf, k = g % M, l = g / M;
+  const cpx<T> *s = src + line_offset(P, l, false);
+  const cpx<T> a = c2r_in(P, s, k), b0 = c2r_in(P, s, M - k);
+  const cpx<T> b{b0.x, -b0.y};
+  const cpx<T> d{a.x - b.x, a.y - b.y};
+  const cpx<T> w = reinterpret_cast<const cpx<T> *>(P.rtw)[k];
+  const cpx<T> wd = cmul(w, d);
+  Z[l * M + k] = cpx<T>{a.x + b.x - wd.y, a.y + b.y + wd.x};   // + i * wd
+}
+
+static int64_t pass_lines(const pfk_pass *p) {
+  int64_t lines = p->text;
+  for (int i = 0; i < p->nb; i++) lines *= p->bext[i];
+  return lines;
+}
+
+// dense line-major layout of `lines` lines of M complex elements in a work
+// buffer, in the pass's line order (tile index innermost)
+static void dense_lines(const pfk_pass *p, int64_t M, int64_t *tst, int64_t *bst) {
+  *tst = M;
+  int64_t w = M * p->text;
+  for (int i = 0; i < p->nb; i++) { bst[i] = w; w *= p->bext[i]; }
+}
+
+static int launch_long(const pfk_pass *p, const void *src, void *dst) {
+  if (p->kind == PFK_C2C) return long_c2c(p, src, dst);
+  if ((p->kind != PFK_R2C && p->kind != PFK_C2R) || !p->work2 || p->n != 2 * p->nf) {
+    set_error_msg("pfk: bad long real-line pass");
+    return 1;
+  }
+  const int64_t M = p->nf, lines = pass_lines(p);
+  pfk_pass c = *p;   // the half-length complex line
+  c.kind = PFK_C2C;
+  c.n = M;
+  c.hw = 0;
+  c.rtw = NULL;
+  c.nin = c.nout = M;
+  c.zin = c.zout = 0;
+  c.imod.on = c.omod.on = 0;
+  hipError_t e;
+  if (p->kind == PFK_R2C) {
+    // real input viewed as complex: contiguous lines, offsets halved
+    c.sin = 1;
+    c.tsin = p->tsin / 2;
+    for (int i = 0; i < p->nb; i++) c.bsin[i] = p->bsin[i] / 2;
+    c.src_elems = p->src_elems / 2;
+    c.sout = 1;
+    dense_lines(p, M, &c.tsout, c.bsout);
+    c.dst_elems = lines * M;
+    if (long_c2c(&c, src, p->work2)) return 1;
+    const int64_t total = lines * p->nout;
+    if (total <= 0) return 0;
+    if (p->prec == PFK_F64)
+      hipLaunchKernelGGL(k_r2c_long_post<double>, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                         0, g_stream, *p, total, (const cpx<double> *)p->work2,
+                         (cpx<double> *)dst);
+    else
+      hipLaunchKernelGGL(k_r2c_long_post<float>, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                         0, g_stream, *p, total, (const cpx<float> *)p->work2, (cpx<float> *)dst);
+    e = hipGetLastError();
+    if (e != hipSuccess) { set_error("pfk r2c long combine", e); return 1; }
+    return 0;
+  }
+  const int64_t total = lines * M;
+  if (total <= 0) return 0;
+  if (p->prec == PFK_F64)
+    hipLaunchKernelGGL(k_c2r_long_pre<double>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       g_stream, *p, total, (const cpx<double> *)src, (cpx<double> *)p->work2);
+  else
+    hipLaunchKernelGGL(k_c2r_long_pre<float>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       g_stream, *p, total, (const cpx<float> *)src, (cpx<float> *)p->work2);
+  e = hipGetLastError();
+  if (e != hipSuccess) { set_error("pfk c2r long combine", e); return 1; }
+  // real output viewed as complex: contiguous lines, offsets halved
+  c.sin = 1;
+  dense_lines(p, M, &c.tsin, c.bsin);
+  c.src_elems = lines * M;
+  c.sout = 1;
+  c.tsout = p->tsout / 2;
+  for (int i = 0; i < p->nb; i++) c.bsout[i] = p->bsout[i] / 2;
+  c.dst_elems = p->dst_elems / 2;
+  return long_c2c(&c, p->work2, dst);
 }
 
 template <typename T>

@@ -93,13 +93,18 @@ typedef struct pfk_pass {
                          workgroup per tile instead of the persistent grid; + 8: line-major
                          lane order (complex64 c2c) */
   int32_t r2r;        /* enum pfk_r2r of a PFK_R2R pass */
-  /* Long lines (c2c, nf > PFK_MAXLINE): nf = n1 * n2 is computed in two
+  /* Long lines (nf > PFK_MAXLINE): nf = n1 * n2 is computed in two
    * steps through `work` (lines * nf complex elements, owned by the plan):
    * length-n1 DFTs over the decimated lines j2, j2 + n2, ... times
    * W_nf^(j2*k1), then length-n2 DFTs (Cooley-Tukey four-step).  The twiddle
-   * table holds W_n1 (n1), W_n2 (n2), W_nf^l (l < 4096) and W_nf^(4096 h). */
+   * table holds W_n1 (n1), W_n2 (n2), W_nf^l (l < 4096) and W_nf^(4096 h).
+   * Real lines (r2c / c2r, n even, nf = n/2; contiguous and unembedded on
+   * the real side) pass through `work2` as the half-length complex line
+   * z_j = x_2j + i x_2j+1 (r2c: transform then combine the half spectrum;
+   * c2r: combine then transform). */
   int64_t n1;         /* 0: the line fits one workgroup */
   void *work;
+  void *work2;        /* real lines (r2c / c2r, nf = n/2): the half-length complex line */
   /* Fields of the two steps (set by the long-line launcher; 0 otherwise):
    * dec_in:  element j of line c is index j*dec_in + c of the original line,
    *          whose embed (nin, zin) and input modulation apply;

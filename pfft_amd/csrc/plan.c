@@ -66,6 +66,7 @@ struct pa_plan {
   void *tw_bufs[4 * PA_MAXRNK];
   int ntw;
   void *work[PA_MAXRNK];         /* long-line passes: lines * nf complex elements */
+  void *work2[PA_MAXRNK];        /* long real-line passes: the same again */
   /* execution timer (kernel/timer.c fields): pass k adds its device time
    * to trafo[tslot[k]]; events tev[0..npass] bracket the passes */
   pa_timer *timer;
@@ -263,7 +264,11 @@ static void free_plan_memory(pa_plan *p) {
   for (int i = 0; i < p->ntw; i++) pfk_free(p->tw_bufs[i]);
   p->ntw = 0;
   for (int i = 0; i < 2; i++) { pfk_free(p->scratch[i]); p->scratch[i] = NULL; }
-  for (int k = 0; k < PA_MAXRNK; k++) { pfk_free(p->work[k]); p->work[k] = NULL; }
+  for (int k = 0; k < PA_MAXRNK; k++) {
+    pfk_free(p->work[k]);
+    pfk_free(p->work2[k]);
+    p->work[k] = p->work2[k] = NULL;
+  }
 }
 
 /* Chooses every pass's launch variant (tune.c) on temporary device
@@ -537,11 +542,12 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     ps->omod.half = no[d] / 2;
     ps->omod.extra = 1;
     if (kind != PFK_COPY && ps->nf > PFK_MAXLINE) {
-      /* long lines: two steps through a work buffer (c2c only) */
-      if (kind != PFK_C2C) {
+      /* long lines: two steps through a work buffer (c2c; even-length
+       * r2c / c2r as their half-length complex line) */
+      if (kind == PFK_R2R || ((kind == PFK_R2C || kind == PFK_C2R) && n[d] % 2)) {
         pa_error("dimension %d: %s lines of computed length %lld exceed %d (supported for c2c "
-                 "dimensions only)", d, kind == PFK_R2R ? "r2r" : "real", (long long)ps->nf,
-                 PFK_MAXLINE);
+                 "and even-length r2c/c2r dimensions)", d, kind == PFK_R2R ? "r2r" : "odd real",
+                 (long long)ps->nf, PFK_MAXLINE);
         goto fail;
       }
       ps->n1 = pfk_long_split(ps->nf);
@@ -574,6 +580,25 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
       int64_t lines = ps->text;
       for (int i = 0; i < ps->nb; i++) lines *= ps->bext[i];
       const size_t wb = (size_t)(lines * ps->nf) * (prec == PFK_F64 ? 16 : 8);
+      if (kind == PFK_R2C || kind == PFK_C2R) {
+        /* the real side is read / written as complex pairs */
+        const int r2c = kind == PFK_R2C;
+        int even = (r2c ? ps->tsin : ps->tsout) % 2 == 0;
+        for (int i = 0; i < ps->nb; i++) even &= (r2c ? ps->bsin[i] : ps->bsout[i]) % 2 == 0;
+        const int plain = r2c ? (ps->sin == 1 && ps->nin == n[d] && ps->zin == 0 && !ps->imod.on)
+                              : (ps->sout == 1 && ps->nout == n[d] && ps->zout == 0 &&
+                                 !ps->omod.on);
+        if (!even || !plain) {
+          pa_error("dimension %d: real lines longer than %d need contiguous, unembedded, "
+                   "unshifted real data (howmany 1)", d, 2 * PFK_MAXLINE);
+          goto fail;
+        }
+        if (pfk_malloc(&p->work2[k], wb)) {
+          pa_error("device allocation of %zu bytes failed: %s", wb, pfk_last_error());
+          goto fail;
+        }
+        ps->work2 = p->work2[k];
+      }
       if (pfk_malloc(&p->work[k], wb)) {
         pa_error("device allocation of %zu bytes failed: %s", wb, pfk_last_error());
         goto fail;

@@ -1,14 +1,17 @@
-"""GPU parity of c2c dimensions longer than one workgroup's line
-(computed length > 4096: two steps through a work buffer, n = n1 * n2)
-against the oracle: power-of-two and mixed lengths, the long dimension
+"""GPU parity of dimensions longer than one workgroup's line (computed
+length > 4096: two steps through a work buffer, n = n1 * n2) against the
+oracle: c2c with power-of-two and mixed lengths, the long dimension
 innermost, outermost and in the middle, transposed layouts, embed / trunc
-and shifts on the long dimension, complex64, in-place, the adjoint, batch
-execution, and the plan-time rejections (no split, long real lines)."""
+and shifts on the long dimension, complex64, in-place, batch execution;
+even-length r2c / c2r real lines (half-length complex line plus the
+half-spectrum combine) in transposed, plain and padded layouts and both
+directions; the adjoints; and the plan-time rejections."""
 import numpy as np
 import pytest
 
 import pfft_amd as P
-from gpu_util import spec_of, random_input, run_gpu, oracle_out, rel_l2, n_scalars, comm_for
+from gpu_util import (spec_of, random_input, run_gpu, oracle_out, rel_l2, n_scalars, comm_for,
+                      out_mask)
 
 pytestmark = pytest.mark.gpu
 
@@ -88,13 +91,59 @@ def test_long_in_batch_execution():
         a.free(); b.free()
 
 
+def check_real(kind, n, flags, sign=-1, rnk_pm=1, prec="f64"):
+    spec = spec_of(kind, n, sign=sign, flags=flags, rnk_pm=rnk_pm)
+    x = random_input(spec, 7)
+    if prec == "f32":
+        x = x.astype(np.float32).astype(np.float64)
+    got = run_gpu(spec, x, prec=prec)
+    want = oracle_out(spec, x)
+    err = rel_l2(got, want, out_mask(spec))
+    assert err <= (1e-12 if prec == "f64" else 2e-6), (kind, n, flags, err)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_long_real_lines(prec):
+    check_real("r2c", [2, 10000], TO, prec=prec)
+    check_real("c2r", [2, 10000], TI, sign=+1, prec=prec)
+    check_real("r2c", [3, 4, 9000], TO, rnk_pm=2, prec=prec)
+    check_real("c2r", [3, 4, 9000], TI, sign=+1, rnk_pm=2, prec=prec)
+
+
+def test_long_real_layouts():
+    check_real("r2c", [2, 16384], 0)
+    check_real("c2r", [2, 16384], 0, sign=+1)
+    check_real("r2c", [2, 12000], P.PADDED_R2C)
+    check_real("c2r", [2, 12000], P.PADDED_C2R, sign=+1)
+    check_real("r2c", [2, 8194], 0, sign=+1)          # conjugated direction, odd half length
+    check_real("c2r", [2, 8194], 0, sign=-1)
+
+
+def test_long_real_adjoint_dot_product():
+    for kind, flags, sign in (("r2c", TO, -1), ("c2r", TI, +1)):
+        n = [2, 10000]
+        spec = spec_of(kind, n, sign=sign, flags=flags, rnk_pm=1)
+        rng = np.random.default_rng(8)
+        x = rng.standard_normal(n_scalars(spec, "in"))
+        y = rng.standard_normal(n_scalars(spec, "out"))
+        if kind == "c2r":
+            # a Hermitian-consistent input: the c2r of real data's spectrum
+            x = oracle_out(spec_of("r2c", n, flags=TO, rnk_pm=1), rng.standard_normal(20000))
+        fx = run_gpu(spec, x)
+        fty = run_gpu(spec, y, adjoint=True)
+        lhs, rhs = float(fx @ y), float(x @ fty)
+        assert abs(lhs - rhs) <= 1e-10 * (abs(lhs) + abs(rhs)), (kind, lhs, rhs)
+
+
 def test_long_rejections():
     c1 = comm_for(1)
     d = P.DeviceBuffer(16 * 2 * 12288)
     assert not P.Plan("c2c", [2, 4099], d.ptr, d.ptr, comm=c1).ok        # prime: no split
     assert not P.Plan("c2c", [2, 2 * 4099], d.ptr, d.ptr, comm=c1).ok    # no split <= 4096
-    assert not P.Plan("r2c", [2, 10000], d.ptr, d.ptr, comm=c1).ok       # long real line
-    assert not P.Plan("c2r", [2, 10000], d.ptr, d.ptr, comm=c1).ok
+    assert not P.Plan("r2c", [2, 10001], d.ptr, d.ptr, comm=c1).ok       # odd long real line
+    assert not P.Plan("c2r", [2, 10001], d.ptr, d.ptr, comm=c1).ok
+    assert not P.Plan("r2c", [2, 10000], d.ptr, d.ptr, ni=[2, 9000], comm=c1).ok   # embedded
+    assert not P.Plan("r2c", [2, 10000], d.ptr, d.ptr, howmany=2, comm=c1).ok     # strided
     pl = P.Plan("c2c", [2, 4096 * 3], d.ptr, d.ptr, comm=c1)
     assert pl.ok and any(ln.endswith("kernel=long-two-step") for ln in pl.describe())
     pl.destroy()
