@@ -271,6 +271,81 @@ k_c2r_long_pre(const pfk_pass P, const int64_t total, const cpx<T> *__restrict__
   Z[l * M + k] = cpx<T>{a.x + b.x - wd.y, a.y + b.y + wd.x};   // + i * wd
 }
 
+// ---- long r2r lines (extended length nf > PFK_MAXLINE) ----
+// u_q of the extension (layout in pfk.h) from the embedded real line: the
+// line index p and sign that land on extension slot q, or p = -1 (zero)
+__device__ __forceinline__ int64_t r2r_source(int kind, int64_t n, int64_t nf, int64_t q,
+                                              int *sgn) {
+  *sgn = 1;
+  switch (kind) {
+    case PFK_DHT: return q;
+    case PFK_REDFT00: return q < n ? q : nf - q;
+    case PFK_REDFT10:
+      if (!(q & 1)) return -1;
+      return q < 2 * n ? (q - 1) / 2 : (nf - q - 1) / 2;
+    case PFK_REDFT01:
+      if (q < n) return q;
+      return q > 3 * n ? nf - q : -1;
+    case PFK_REDFT11:
+      if (!(q & 1)) return -1;
+      if (q < 2 * n) return (q - 1) / 2;
+      return q > 6 * n ? (nf - q - 1) / 2 : -1;
+    case PFK_RODFT00:
+      if (q >= 1 && q <= n) return q - 1;
+      *sgn = -1;
+      return (q >= n + 2) ? nf - q - 1 : -1;
+    case PFK_RODFT10:
+      if (!(q & 1)) return -1;
+      if (q < 2 * n) return (q - 1) / 2;
+      *sgn = -1;
+      return (nf - q - 1) / 2;
+    case PFK_RODFT01:
+      if (q >= 1 && q <= n) return q - 1;
+      *sgn = -1;
+      return q > 3 * n ? nf - q - 1 : -1;
+    default:   // RODFT11
+      if (!(q & 1)) return -1;
+      if (q < 2 * n) return (q - 1) / 2;
+      *sgn = -1;
+      return q > 6 * n ? (nf - q - 1) / 2 : -1;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_r2r_long_pre(const pfk_pass P, const int64_t total, const T *__restrict__ src,
+               cpx<T> *__restrict__ U) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= total) return;
+  const int64_t nf = P.nf, q = g % nf, l = g / nf;
+  int sgn;
+  const int64_t p = r2r_source(P.r2r, P.n, nf, q, &sgn);
+  const int64_t j = p - P.zin;   // index into the embedded input line
+  T v = 0;
+  if (p >= 0 && j >= 0 && j < P.nin)
+    v = src[line_offset(P, l, false) + j * P.sin] * (T)(sgn * mod_sign(P.imod, j));
+  U[g] = cpx<T>{v, (T)0};
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_r2r_long_post(const pfk_pass P, const int64_t total, const cpx<T> *__restrict__ F,
+                T *__restrict__ dst) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= total) return;
+  const int64_t k = g % P.nout, l = g / P.nout, kk = P.zout + k;
+  const cpx<T> *f = F + l * P.nf;
+  T y;
+  switch (P.r2r) {
+    case PFK_DHT: y = f[kk].x - f[kk].y; break;
+    case PFK_REDFT00: case PFK_REDFT10: y = f[kk].x; break;
+    case PFK_REDFT01: case PFK_REDFT11: y = f[2 * kk + 1].x; break;
+    case PFK_RODFT00: case PFK_RODFT10: y = -f[kk + 1].y; break;
+    default: y = -f[2 * kk + 1].y; break;   // RODFT01, RODFT11
+  }
+  dst[line_offset(P, l, true) + k * P.sout] = y * (T)mod_sign(P.omod, k);
+}
+
 static int64_t pass_lines(const pfk_pass *p) {
   int64_t lines = p->text;
   for (int i = 0; i < p->nb; i++) lines *= p->bext[i];
@@ -285,8 +360,47 @@ static void dense_lines(const pfk_pass *p, int64_t M, int64_t *tst, int64_t *bst
   for (int i = 0; i < p->nb; i++) { bst[i] = w; w *= p->bext[i]; }
 }
 
+// r2r: extension into work2, the long c2c in place on work2, output picks
+static int long_r2r(const pfk_pass *p, const void *src, void *dst) {
+  const int64_t nf = p->nf, lines = pass_lines(p);
+  if (!p->work2) { set_error_msg("pfk: bad long r2r pass"); return 1; }
+  const int64_t tot = lines * nf;
+  if (tot <= 0) return 0;
+  if (p->prec == PFK_F64)
+    hipLaunchKernelGGL(k_r2r_long_pre<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       g_stream, *p, tot, (const double *)src, (cpx<double> *)p->work2);
+  else
+    hipLaunchKernelGGL(k_r2r_long_pre<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       g_stream, *p, tot, (const float *)src, (cpx<float> *)p->work2);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_error("pfk r2r long extension", e); return 1; }
+  pfk_pass c = *p;
+  c.kind = PFK_C2C;
+  c.n = nf;
+  c.nin = c.nout = nf;
+  c.zin = c.zout = 0;
+  c.imod.on = c.omod.on = 0;
+  c.sin = c.sout = 1;
+  dense_lines(p, nf, &c.tsin, c.bsin);
+  dense_lines(p, nf, &c.tsout, c.bsout);
+  c.src_elems = c.dst_elems = tot;
+  if (long_c2c(&c, p->work2, p->work2)) return 1;   // step A reads work2 before step B writes it
+  const int64_t out = lines * p->nout;
+  if (out <= 0) return 0;
+  if (p->prec == PFK_F64)
+    hipLaunchKernelGGL(k_r2r_long_post<double>, dim3((unsigned)((out + 255) / 256)), dim3(256), 0,
+                       g_stream, *p, out, (const cpx<double> *)p->work2, (double *)dst);
+  else
+    hipLaunchKernelGGL(k_r2r_long_post<float>, dim3((unsigned)((out + 255) / 256)), dim3(256), 0,
+                       g_stream, *p, out, (const cpx<float> *)p->work2, (float *)dst);
+  e = hipGetLastError();
+  if (e != hipSuccess) { set_error("pfk r2r long output", e); return 1; }
+  return 0;
+}
+
 static int launch_long(const pfk_pass *p, const void *src, void *dst) {
   if (p->kind == PFK_C2C) return long_c2c(p, src, dst);
+  if (p->kind == PFK_R2R) return long_r2r(p, src, dst);
   if ((p->kind != PFK_R2C && p->kind != PFK_C2R) || !p->work2 || p->n != 2 * p->nf) {
     set_error_msg("pfk: bad long real-line pass");
     return 1;

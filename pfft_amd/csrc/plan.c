@@ -344,10 +344,6 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
         pa_error("r2r kind %d of dimension %d is not supported (DHT, REDFT, RODFT)", kinds[t], t);
         return NULL;
       }
-      if (m > 4096) {   /* the general kernel keeps a line in one workgroup */
-        pa_error("r2r dimension %d: extended length %lld exceeds one workgroup", t, (long long)m);
-        return NULL;
-      }
     }
   }
   if ((pfft_flags & PFFT_AMD_MFMA_F32) && (pfft_flags & PFFT_AMD_MFMA_BF16X2)) {
@@ -544,9 +540,8 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     if (kind != PFK_COPY && ps->nf > PFK_MAXLINE) {
       /* long lines: two steps through a work buffer (c2c; even-length
        * r2c / c2r as their half-length complex line) */
-      if (kind == PFK_R2R || ((kind == PFK_R2C || kind == PFK_C2R) && n[d] % 2)) {
-        pa_error("dimension %d: %s lines of computed length %lld exceed %d (supported for c2c "
-                 "and even-length r2c/c2r dimensions)", d, kind == PFK_R2R ? "r2r" : "odd real",
+      if ((kind == PFK_R2C || kind == PFK_C2R) && n[d] % 2) {
+        pa_error("dimension %d: odd real lines of computed length %lld exceed %d", d,
                  (long long)ps->nf, PFK_MAXLINE);
         goto fail;
       }
@@ -580,7 +575,14 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
       int64_t lines = ps->text;
       for (int i = 0; i < ps->nb; i++) lines *= ps->bext[i];
       const size_t wb = (size_t)(lines * ps->nf) * (prec == PFK_F64 ? 16 : 8);
-      if (kind == PFK_R2C || kind == PFK_C2R) {
+      if (kind == PFK_R2R) {
+        /* the extended line lives in work2 */
+        if (pfk_malloc(&p->work2[k], wb)) {
+          pa_error("device allocation of %zu bytes failed: %s", wb, pfk_last_error());
+          goto fail;
+        }
+        ps->work2 = p->work2[k];
+      } else if (kind == PFK_R2C || kind == PFK_C2R) {
         /* the real side is read / written as complex pairs */
         const int r2c = kind == PFK_R2C;
         int even = (r2c ? ps->tsin : ps->tsout) % 2 == 0;

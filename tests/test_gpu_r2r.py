@@ -11,6 +11,7 @@ from gpu_util import spec_of, random_input, run_gpu, oracle_out, rel_l2, n_scala
 pytestmark = pytest.mark.gpu
 
 TI, TO = P.TRANSPOSED_IN, P.TRANSPOSED_OUT
+SI, SO = P.SHIFTED_IN, P.SHIFTED_OUT
 KINDS = [P.DHT, P.REDFT00, P.REDFT01, P.REDFT10, P.REDFT11,
          P.RODFT00, P.RODFT01, P.RODFT10, P.RODFT11]
 
@@ -98,8 +99,24 @@ def test_rejections():
     assert not P.Plan("r2r", [4, 4, 4], d.ptr, d.ptr, comm=comm, kinds=[0, 3, 3]).ok   # R2HC
     assert not P.Plan("r2r", [4, 4, 5], d.ptr, d.ptr, comm=comm, kinds=[3, 3, 3],
                       flags=P.SHIFTED_IN | P.SHIFTED_OUT).ok                         # odd n
-    assert not P.Plan("r2r", [4, 4, 1024], d.ptr, d.ptr, comm=comm, kinds=[3, 3, 6]).ok
+    assert not P.Plan("r2r", [4, 4, 4099], d.ptr, d.ptr, comm=comm, kinds=[3, 3, 2]).ok  # no split
     d.free()
+
+
+@pytest.mark.parametrize("kind,n", [(P.REDFT10, 2048), (P.REDFT01, 1500), (P.REDFT11, 1200),
+                                    (P.RODFT10, 1100), (P.RODFT01, 2048), (P.RODFT11, 640),
+                                    (P.REDFT00, 5001), (P.RODFT00, 4999), (P.DHT, 6000)])
+def test_long_extended_lines(kind, n):
+    """Extended length above one workgroup's line: the extension is built
+    in a work buffer and transformed in two steps."""
+    check([2, n, 2], [P.DHT, kind, P.REDFT10], rnk_pm=1)
+
+
+def test_long_extended_lines_layouts():
+    check([3, 2048, 2], [P.REDFT10, P.REDFT10, P.RODFT00], ni=[3, 1500, 2], no=[2, 1800, 2])
+    check([4, 1024, 2], [P.RODFT11, P.REDFT11, P.DHT], flags=SO | SI)
+    check([2048, 3, 2], [P.REDFT10, P.DHT, P.RODFT10], flags=TO)
+    check([2, 2, 1024], [P.DHT, P.REDFT01, P.REDFT11], prec="f32")
 
 
 def test_shortest_lines():
