@@ -360,6 +360,229 @@ static void dense_lines(const pfk_pass *p, int64_t M, int64_t *tst, int64_t *bst
   for (int i = 0; i < p->nb; i++) { bst[i] = w; w *= p->bext[i]; }
 }
 
+// ---- r2r through an inner FFT (r2r_path 1, 2; modes in pfk.h) ----
+// element j of the embedded, modulated input line (0 outside the embed)
+template <typename T>
+__device__ __forceinline__ T r2r_x(const pfk_pass &P, const T *s, int64_t j) {
+  const int64_t i = j - P.zin;
+  if (j < 0 || i < 0 || i >= P.nin) return (T)0;
+  return s[i * P.sin] * (T)mod_sign(P.imod, i);
+}
+
+// mode 0: element m of the real line whose real FFT the kind needs
+template <typename T>
+__device__ __forceinline__ T r2r_v(const pfk_pass &P, const T *s, int64_t m) {
+  const int64_t n = P.n;
+  switch (P.r2r) {
+    case PFK_DHT: return r2r_x(P, s, m);
+    case PFK_REDFT10: case PFK_RODFT10: {   // Makhoul: v_m = x_2m, v_(n-1-m') = x_(2m'+1)
+      const int64_t j = m < (n + 1) / 2 ? 2 * m : 2 * (n - 1 - m) + 1;
+      const T v = r2r_x(P, s, j);
+      return (P.r2r == PFK_RODFT10 && (j & 1)) ? -v : v;   // DST-II: x_j (-1)^j
+    }
+    case PFK_REDFT00: return r2r_x(P, s, m < n ? m : 2 * (n - 1) - m);
+    default:   // RODFT00
+      if (m >= 1 && m <= n) return r2r_x(P, s, m - 1);
+      return (m >= n + 2) ? -r2r_x(P, s, 2 * (n + 1) - m - 1) : (T)0;
+  }
+}
+
+// mode 1: V_j = e^(i pi j/(2n)) (x'_j - i x'_(n-j)), x'_n = 0, j <= n/2
+// (x' = x, reversed for DST-III); Im of V_0 and V_(n/2) dropped as a c2r
+// drops them
+template <typename T>
+__device__ __forceinline__ cpx<T> r2r_V(const pfk_pass &P, const T *s, int64_t j) {
+  const int64_t n = P.n;
+  const bool rev = P.r2r == PFK_RODFT01;
+  const T a = r2r_x(P, s, rev ? n - 1 - j : j);
+  const T b = (j == 0) ? (T)0 : r2r_x(P, s, rev ? j - 1 : n - j);
+  const cpx<T> t = reinterpret_cast<const cpx<T> *>(P.tw2)[j];   // e^(-i pi j/(2n))
+  cpx<T> v = cmul(cpx<T>{t.x, -t.y}, cpx<T>{a, -b});
+  if (j == 0 || 2 * j == n) v.y = 0;
+  return v;
+}
+
+// complex index of element q of line l in a work array
+__device__ __forceinline__ int64_t widx(int lm, int64_t l, int64_t q, int64_t lines,
+                                        int64_t M) {
+  return lm ? l * M + q : q * lines + l;
+}
+
+// path 1, pre: element q < M of the inner complex line of line l
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_r2r_pre1(const pfk_pass P, const int64_t lines, const int64_t M, const int lm,
+           const T *__restrict__ src, cpx<T> *__restrict__ w1) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= lines * M) return;
+  const int64_t q = lm ? g % M : g / lines, l = lm ? g / M : g % lines, n = P.n;
+  const T *s = src + line_offset(P, l, false);
+  const int mode = pfk_r2r_mode(P.r2r);
+  cpx<T> z;
+  if (mode == 0) {
+    z = cpx<T>{r2r_v(P, s, 2 * q), r2r_v(P, s, 2 * q + 1)};
+  } else if (mode == 1) {   // c2r combine: (V_q + conj V_(M-q)) + i W_n^q (V_q - conj V_(M-q))
+    const cpx<T> a = r2r_V(P, s, q), b0 = r2r_V(P, s, M - q);
+    const cpx<T> b{b0.x, -b0.y};
+    const cpx<T> wd = cmul(reinterpret_cast<const cpx<T> *>(P.rtw)[q], cpx<T>{a.x - b.x, a.y - b.y});
+    z = cpx<T>{a.x + b.x - wd.y, a.y + b.y + wd.x};
+  } else {   // z_q = (x'_2q + i x'_(n-1-2q)) W_8n^(4q+1), x' reversed for DST-IV
+    const bool rev = P.r2r == PFK_RODFT11;
+    const T a = r2r_x(P, s, rev ? n - 1 - 2 * q : 2 * q);
+    const T b = r2r_x(P, s, rev ? 2 * q : n - 1 - 2 * q);
+    z = cmul(cpx<T>{a, b}, reinterpret_cast<const cpx<T> *>(P.tw2)[4 * q + 1]);
+  }
+  w1[widx(lm, l, q, lines, M)] = z;
+}
+
+// path 1, post: kept output k of line l from the inner result Z
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_r2r_post1(const pfk_pass P, const int64_t lines, const int64_t M, const int lm,
+            const cpx<T> *__restrict__ Z, T *__restrict__ dst) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= lines * P.nout) return;
+  const int64_t k = lm ? g % P.nout : g / lines, l = lm ? g / P.nout : g % lines;
+  const int64_t n = P.n, kk = P.zout + k;
+  const int mode = pfk_r2r_mode(P.r2r);
+  const cpx<T> *tw2 = reinterpret_cast<const cpx<T> *>(P.tw2);
+  auto z = [&](int64_t m) { return Z[widx(lm, l, m, lines, M)]; };
+  T y;
+  if (mode == 0) {
+    // V_m of the real line (length L = 2M): E_m + W_L^m O_m from Z_m and
+    // conj Z_(M-m); for m > M the conjugate of V_(L-m)
+    auto V = [&](int64_t m) -> cpx<T> {
+      const bool up = m > M;
+      const int64_t mm = up ? 2 * M - m : m;
+      const cpx<T> a = z(mm % M), b0 = z((M - mm) % M);
+      const cpx<T> b{b0.x, -b0.y};
+      const cpx<T> e{(a.x + b.x) * (T)0.5, (a.y + b.y) * (T)0.5};
+      const cpx<T> o{(a.y - b.y) * (T)0.5, -(a.x - b.x) * (T)0.5};
+      cpx<T> v = cmul(reinterpret_cast<const cpx<T> *>(P.rtw)[mm], o);
+      v = cpx<T>{v.x + e.x, v.y + e.y};
+      return up ? cpx<T>{v.x, -v.y} : v;
+    };
+    switch (P.r2r) {
+      case PFK_DHT: { const cpx<T> v = V(kk); y = v.x - v.y; break; }
+      case PFK_REDFT10: case PFK_RODFT10: {
+        const int64_t m = P.r2r == PFK_RODFT10 ? n - 1 - kk : kk;   // DST-II: reversed
+        y = 2 * cmul(tw2[m], V(m)).x;
+        break;
+      }
+      case PFK_REDFT00: y = V(kk).x; break;
+      default: y = -V(kk + 1).y; break;   // RODFT00
+    }
+  } else if (mode == 1) {   // y_2m = v_m, y_(2m+1) = v_(n-1-m); v_2j + i v_2j+1 = Z_j
+    const int64_t idx = (kk & 1) ? n - 1 - (kk - 1) / 2 : kk / 2;
+    const cpx<T> c = z(idx / 2);
+    y = (idx & 1) ? c.y : c.x;
+    if (P.r2r == PFK_RODFT01 && (kk & 1)) y = -y;   // DST-III: (-1)^k
+  } else {   // u_j = Z_j W_8n^(4j); y_2j = 2 Re u_j, y_(n-1-2j) = -2 Im u_j
+    const int64_t j = (kk & 1) ? (n - 1 - kk) / 2 : kk / 2;
+    const cpx<T> u = cmul(z(j), tw2[4 * j]);
+    y = (kk & 1) ? -2 * u.y : 2 * u.x;
+    if (P.r2r == PFK_RODFT11 && (kk & 1)) y = -y;   // DST-IV: (-1)^k
+  }
+  dst[line_offset(P, l, true) + k * P.sout] = y * (T)mod_sign(P.omod, k);
+}
+
+// path 2 (odd lengths), pre: one element of the inner r2c / c2r input line
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_r2r_pre2(const pfk_pass P, const int64_t total, const int64_t q_per_line, const int64_t ld1,
+           const T *__restrict__ src, T *__restrict__ w1) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= total) return;
+  const int64_t q = g % q_per_line, l = g / q_per_line;
+  const T *s = src + line_offset(P, l, false);
+  T *o = w1 + l * ld1;
+  if (pfk_r2r_mode(P.r2r) == 0) {
+    o[q] = r2r_v(P, s, q);
+  } else {   // the half spectrum V_j, j <= n/2, of the inverse Makhoul step
+    const cpx<T> v = r2r_V(P, s, q);
+    o[2 * q] = v.x;
+    o[2 * q + 1] = v.y;
+  }
+}
+
+// path 2, post: one kept output from the inner r2c / c2r result
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_r2r_post2(const pfk_pass P, const int64_t total, const int64_t ld2, const T *__restrict__ w2,
+            T *__restrict__ dst) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= total) return;
+  const int64_t k = g % P.nout, l = g / P.nout, n = P.n;
+  const int64_t kk = P.zout + k;
+  const T *w = w2 + l * ld2;
+  const cpx<T> *F = reinterpret_cast<const cpx<T> *>(w);
+  const cpx<T> *tw2 = reinterpret_cast<const cpx<T> *>(P.tw2);
+  const int64_t L = pfk_r2r_inner_len(P.r2r, n);
+  auto half = [&](int64_t m) -> cpx<T> {
+    return m <= L / 2 ? F[m] : cpx<T>{F[L - m].x, -F[L - m].y};
+  };
+  T y;
+  switch (P.r2r) {
+    case PFK_DHT: { const cpx<T> f = half(kk); y = f.x - f.y; break; }
+    case PFK_REDFT10: case PFK_RODFT10: {
+      const int64_t m = P.r2r == PFK_RODFT10 ? n - 1 - kk : kk;
+      y = 2 * cmul(tw2[m], half(m)).x;
+      break;
+    }
+    case PFK_REDFT00: y = F[kk].x; break;
+    case PFK_RODFT00: y = -F[kk + 1].y; break;
+    default: {   // REDFT01, RODFT01
+      const T v = w[(kk & 1) ? n - 1 - (kk - 1) / 2 : kk / 2];
+      y = (P.r2r == PFK_RODFT01 && (kk & 1)) ? -v : v;
+      break;
+    }
+  }
+  dst[line_offset(P, l, true) + k * P.sout] = y * (T)mod_sign(P.omod, k);
+}
+
+template <typename T>
+static int r2r_real_path_t(const pfk_pass *p, const void *src, void *dst) {
+  pfk_pass in;
+  int64_t ld1, ld2;
+  int path;
+  if (pfk_r2r_inner(p, &in, &ld1, &ld2, &path) || !p->work || !p->work2) {
+    set_error_msg("pfk: bad r2r pass");
+    return 1;
+  }
+  const int64_t lines = pass_lines(p), out = lines * p->nout;
+  if (lines <= 0 || out <= 0) return 0;
+  const dim3 blk(256);
+  if (path == 1) {
+    const int64_t M = in.nf;
+    const int lm1 = p->sin == 1, lm2 = p->sout == 1;
+    hipLaunchKernelGGL(k_r2r_pre1<T>, dim3((unsigned)((lines * M + 255) / 256)), blk, 0,
+                       g_stream, *p, lines, M, lm1, (const T *)src, (cpx<T> *)p->work);
+  } else {
+    const int mode = pfk_r2r_mode(p->r2r);
+    const int64_t qpl = mode == 0 ? pfk_r2r_inner_len(p->r2r, p->n) : p->n / 2 + 1;
+    hipLaunchKernelGGL(k_r2r_pre2<T>, dim3((unsigned)((lines * qpl + 255) / 256)), blk, 0,
+                       g_stream, *p, lines * qpl, qpl, ld1, (const T *)src, (T *)p->work);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_error("pfk r2r pre", e); return 1; }
+  if (pfk_launch_pass(&in, p->work, p->work2)) return 1;
+  if (path == 1)
+    hipLaunchKernelGGL(k_r2r_post1<T>, dim3((unsigned)((out + 255) / 256)), blk, 0, g_stream, *p,
+                       lines, (int64_t)in.nf, (int)(p->sout == 1), (const cpx<T> *)p->work2,
+                       (T *)dst);
+  else
+    hipLaunchKernelGGL(k_r2r_post2<T>, dim3((unsigned)((out + 255) / 256)), blk, 0, g_stream, *p,
+                       out, ld2, (const T *)p->work2, (T *)dst);
+  e = hipGetLastError();
+  if (e != hipSuccess) { set_error("pfk r2r post", e); return 1; }
+  return 0;
+}
+
+static int r2r_real_path(const pfk_pass *p, const void *src, void *dst) {
+  return p->prec == PFK_F64 ? r2r_real_path_t<double>(p, src, dst)
+                            : r2r_real_path_t<float>(p, src, dst);
+}
+
 // r2r: extension into work2, the long c2c in place on work2, output picks
 static int long_r2r(const pfk_pass *p, const void *src, void *dst) {
   const int64_t nf = p->nf, lines = pass_lines(p);
@@ -559,6 +782,66 @@ int pfk_sync(void) {
   return 0;
 }
 
+int pfk_r2r_inner(const pfk_pass *p, pfk_pass *in, int64_t *ld1, int64_t *ld2, int *path) {
+  if (p->kind != PFK_R2R) return 1;
+  const int mode = pfk_r2r_mode(p->r2r);
+  const int64_t n = p->n, L = pfk_r2r_inner_len(p->r2r, n);
+  if (p->r2r == PFK_REDFT00 && n < 2) return 1;
+  const int even = mode == 0 ? L % 2 == 0 : n % 2 == 0;
+  if (mode == 2 && !even) return 1;   // the n/2-point form needs even n
+  int64_t lines = p->text;
+  for (int i = 0; i < p->nb; i++) lines *= p->bext[i];
+  memset(in, 0, sizeof(*in));
+  in->prec = p->prec;
+  in->xcd_remap = p->xcd_remap;
+  in->tw = p->tw;
+  in->text = lines;
+  in->nb = 0;
+  if (even) {
+    // path 1: complex FFT of length M; each work array line-major where the
+    // pass's lines are contiguous on that side, else lines innermost
+    const int64_t M = mode == 0 ? L / 2 : n / 2;
+    *path = 1;
+    in->kind = PFK_C2C;
+    in->sign = mode == 1 ? +1 : -1;
+    in->n = in->nf = in->nin = in->nout = M;
+    const int lm1 = p->sin == 1, lm2 = p->sout == 1;
+    in->sin = lm1 ? 1 : lines;
+    in->tsin = lm1 ? M : 1;
+    in->sout = lm2 ? 1 : lines;
+    in->tsout = lm2 ? M : 1;
+    *ld1 = *ld2 = 2 * M;
+    in->src_elems = in->dst_elems = lines * M;
+  } else {
+    // path 2: r2c / c2r of the odd length, line-major
+    *path = 2;
+    in->sin = in->sout = 1;
+    if (mode == 0) {
+      in->kind = PFK_R2C;
+      in->sign = -1;
+      in->n = in->nin = in->nf = L;
+      in->nout = L / 2 + 1;
+      *ld1 = L + 1;                     // even pitch: complex pairs stay aligned
+      *ld2 = 2 * (L / 2 + 1);
+      in->tsin = *ld1;
+      in->tsout = *ld2 / 2;
+    } else {
+      in->kind = PFK_C2R;
+      in->sign = +1;
+      in->n = in->nout = in->nf = n;
+      in->nin = n / 2 + 1;
+      *ld1 = 2 * (n / 2 + 1);
+      *ld2 = n + 1;
+      in->tsin = *ld1 / 2;
+      in->tsout = *ld2;
+    }
+    in->src_elems = lines * (in->kind == PFK_R2C ? *ld1 : *ld1 / 2);
+    in->dst_elems = lines * (in->kind == PFK_C2R ? *ld2 : *ld2 / 2);
+  }
+  in->nstages = pfk_factorize(in->nf, in->radix);
+  return in->nstages < 0;
+}
+
 int pfk_pass_check(const pfk_pass *p) {
   if (pfk_init()) return 1;
   if (!g_generic_only && (fast_covers(p) || small_covers(p) || mfma_covers(p))) return 0;
@@ -569,6 +852,7 @@ int pfk_pass_check(const pfk_pass *p) {
 
 int pfk_launch_pass(const pfk_pass *p, const void *src, void *dst) {
   if (pfk_init()) return 1;
+  if (p->kind == PFK_R2R && p->r2r_path) return r2r_real_path(p, src, dst);
   if (p->n1) return launch_long(p, src, dst);
   if (!g_generic_only && p->mfma) {
     const int m = launch_mfma(p, src, dst);

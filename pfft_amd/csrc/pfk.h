@@ -34,6 +34,13 @@
 extern "C" {
 #endif
 
+/* helpers below are shared by the C host code and the HIP kernels */
+#ifdef __HIPCC__
+#define PFK_HD __host__ __device__
+#else
+#define PFK_HD
+#endif
+
 #define PFK_MAXB 6
 #define PFK_MAXSTAGES 24
 
@@ -113,6 +120,11 @@ typedef struct pfk_pass {
    * twn:     output k of line c is multiplied by W_twn^(c*k) (table tw2). */
   int64_t dec_in, dec_out, twn;
   const void *tw2;
+  /* r2r passes: 0 = transform of the extension (table above); 1, 2 =
+   * inner FFT of about the line's own length between elementwise pre and
+   * post steps (pfk_r2r_inner), through `work` and `work2` */
+  int32_t r2r_path;
+  int32_t pad3;
   /* device twiddle tables (owned by the host plan):
    *   tw  : W_nf^m = exp(sign*2*pi*i*m/nf), m in [0,nf)
    *   rtw : W_n^k  = exp(sign*2*pi*i*k/n),  k in [0,n/2] (even r2c/c2r only) */
@@ -216,6 +228,36 @@ const char *pfk_last_error(void);
 
 /* Launch one pass on the library stream (asynchronous). */
 int  pfk_launch_pass(const pfk_pass *p, const void *src, void *dst);
+/* r2r kinds computed through an inner FFT of about the line's own length
+ * (r2r_path 1 and 2), between elementwise pre and post steps:
+ *   mode 0  DHT, REDFT10, RODFT10, REDFT00, RODFT00: the real FFT of a real
+ *           line of length L = n, n, n, 2(n-1), 2(n+1) (Makhoul's
+ *           permutation for the 10 kinds, the short symmetric extension for
+ *           the 00 kinds);
+ *   mode 1  REDFT01, RODFT01: the inverse real FFT of length n (inverse
+ *           Makhoul);
+ *   mode 2  REDFT11, RODFT11: a complex FFT of length n/2 with pre and post
+ *           twiddles (n even).
+ * Path 1 (even L, even n): the inner transform is always the complex FFT
+ * of length M = L/2 (or n/2); the real packing z_j = v_2j + i v_2j+1 and
+ * the half-spectrum combine are done in the pre/post steps.  Each work
+ * array is stored line-major when the pass's lines are contiguous on that
+ * side, else with the lines innermost, and the inner pass transposes
+ * between the two.  Path 2 (odd lengths, modes 0 and 1): the inner r2c /
+ * c2r of length L or n, line-major.
+ * Tables: tw = the inner complex FFT; rtw = W_L^m (mode 0) or W_n^m
+ * (mode 1), m <= M, path 1 only; tw2 = W_4n^k, k <= n (10/01 kinds) or
+ * W_8n^m, m < 2n (11 kinds). */
+PFK_HD static inline int pfk_r2r_mode(int kind) {
+  return (kind == 4 || kind == 8) ? 1 : (kind == 6 || kind == 10) ? 2 : 0;   /* 01, 11 kinds */
+}
+PFK_HD static inline int64_t pfk_r2r_inner_len(int kind, int64_t n) {
+  return kind == 3 ? 2 * (n - 1) : kind == 7 ? 2 * (n + 1) : n;   /* REDFT00, RODFT00 */
+}
+/* The inner pass of an r2r pass, its path (1 or 2) and the work arrays'
+ * line pitches in reals; nonzero if the kind / length has no such path. */
+int  pfk_r2r_inner(const pfk_pass *p, pfk_pass *inner, int64_t *ld1, int64_t *ld2, int *path);
+
 /* 0 if a kernel can run the (one-workgroup-per-line) pass: one of the
  * specialised families covers it or the general kernel's line fits the
  * device's LDS; else nonzero with the reason in pfk_last_error(). */

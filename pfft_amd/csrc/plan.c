@@ -67,6 +67,7 @@ struct pa_plan {
   int ntw;
   void *work[PA_MAXRNK];         /* long-line passes: lines * nf complex elements */
   void *work2[PA_MAXRNK];        /* long real-line passes: the same again */
+  void *r2rw[2];                 /* r2r passes on the inner real FFT: its input / output */
   /* execution timer (kernel/timer.c fields): pass k adds its device time
    * to trafo[tslot[k]]; events tev[0..npass] bracket the passes */
   pa_timer *timer;
@@ -269,6 +270,7 @@ static void free_plan_memory(pa_plan *p) {
     pfk_free(p->work2[k]);
     p->work[k] = p->work2[k] = NULL;
   }
+  for (int i = 0; i < 2; i++) { pfk_free(p->r2rw[i]); p->r2rw[i] = NULL; }
 }
 
 /* Chooses every pass's launch variant (tune.c) on temporary device
@@ -458,6 +460,7 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
   int done[PA_MAXRNK] = {0};
   arrgeo cur = p->gin, nxt;
   int64_t scratch_elems[2] = {0, 0};
+  int64_t r2r_elems[2] = {0, 0};   /* reals of the inner-FFT work arrays */
   p->npass = R;
   int src_role = ROLE_IN;
   arrgeo prev_inter;
@@ -537,7 +540,21 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     ps->omod.start = p->local_o_start[d];
     ps->omod.half = no[d] / 2;
     ps->omod.extra = 1;
-    if (kind != PFK_COPY && ps->nf > PFK_MAXLINE) {
+    if (kind == PFK_R2R && n[d] >= 8) {
+      /* the inner real FFT of about the line's own length, where one
+       * workgroup transforms it (else the extension below) */
+      pfk_pass t = *ps, in;
+      int64_t l1, l2;
+      int path;
+      t.n = n[d];
+      t.text = 1;
+      t.nb = 0;
+      t.sin = t.sout = 1;
+      if (!pfk_r2r_inner(&t, &in, &l1, &l2, &path) && in.nf <= PFK_MAXLINE &&
+          !pfk_pass_check(&in))
+        ps->r2r_path = path;
+    }
+    if (kind != PFK_COPY && !ps->r2r_path && ps->nf > PFK_MAXLINE) {
       /* long lines: two steps through a work buffer (c2c; even-length
        * r2c / c2r as their half-length complex line) */
       if ((kind == PFK_R2C || kind == PFK_C2R) && n[d] % 2) {
@@ -555,7 +572,28 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     if (kind != PFK_COPY) {
       ps->nstages = pfk_factorize(ps->nf, ps->radix);
       if (ps->nstages < 0) { pa_error("cannot factor length %lld", (long long)ps->nf); goto fail; }
-      if (ps->n1) {
+      if (ps->r2r_path) {
+        /* tables of the inner transform and the pre/post twiddles (pfk.h) */
+        pfk_pass in;
+        int64_t l1, l2;
+        int path;
+        (void)pfk_r2r_inner(ps, &in, &l1, &l2, &path);
+        ps->tw = make_twiddles(p, in.nf, in.nf, in.sign);
+        p->tw_adj[k] = ps->tw;   /* r2r plans have no adjoint */
+        const int mode = pfk_r2r_mode(kinds[d]);
+        if (path == 1 && mode != 2) {
+          /* the half-spectrum combine: W_L^m (mode 0) or W_n^m (mode 1), m <= M */
+          const INT L = mode == 0 ? pfk_r2r_inner_len(kinds[d], n[d]) : n[d];
+          ps->rtw = make_twiddles(p, L, L / 2 + 1, in.sign);
+          if (!ps->rtw) goto fail;
+        }
+        const int k10 = kinds[d] == PFK_REDFT10 || kinds[d] == PFK_RODFT10 ||
+                        kinds[d] == PFK_REDFT01 || kinds[d] == PFK_RODFT01;
+        const int k11 = kinds[d] == PFK_REDFT11 || kinds[d] == PFK_RODFT11;
+        if (k10) ps->tw2 = make_twiddles(p, 4 * n[d], n[d] + 1, -1);
+        if (k11) ps->tw2 = make_twiddles(p, 8 * n[d], 2 * n[d], -1);
+        if ((k10 || k11) && !ps->tw2) goto fail;
+      } else if (ps->n1) {
         ps->tw = make_long_twiddles(p, ps->nf, ps->n1, ps->sign);
         p->tw_adj[k] = make_long_twiddles(p, ps->nf, ps->n1, -ps->sign);
       } else {
@@ -564,7 +602,7 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
       }
       if (!ps->tw || !p->tw_adj[k]) goto fail;
       if ((kind == PFK_R2C || kind == PFK_C2R) && n[d] % 2 == 0) {
-        ps->rtw = make_twiddles(p, n[d], n[d] / 2 + 1, sign);
+        ps->rtw = make_twiddles(p, n[d], n[d] / 2 + 1, sign);   /* never r2r */
         p->rtw_adj[k] = make_twiddles(p, n[d], n[d] / 2 + 1, -sign);
         if (!ps->rtw || !p->rtw_adj[k]) goto fail;
       }
@@ -606,6 +644,15 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
         goto fail;
       }
       ps->work = p->work[k];
+    } else if (ps->r2r_path) {
+      pfk_pass in;
+      int64_t l1, l2;
+      int path;
+      (void)pfk_r2r_inner(ps, &in, &l1, &l2, &path);
+      int64_t lines = ps->text;
+      for (int i = 0; i < ps->nb; i++) lines *= ps->bext[i];
+      if (lines * l1 > r2r_elems[0]) r2r_elems[0] = lines * l1;
+      if (lines * l2 > r2r_elems[1]) r2r_elems[1] = lines * l2;
     } else if (pfk_pass_check(ps)) {
       pa_error("dimension %d: %s", d, pfk_last_error());
       goto fail;
@@ -629,6 +676,15 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
     src_role = dst_role;
     cur = nxt;
   }
+  for (int i = 0; i < 2; i++) {
+    const size_t b = (size_t)r2r_elems[i] * (prec == PFK_F64 ? 8 : 4);
+    if (b && pfk_malloc(&p->r2rw[i], b)) {
+      pa_error("device allocation of %zu bytes failed: %s", b, pfk_last_error());
+      goto fail;
+    }
+  }
+  for (int k = 0; k < R; k++)
+    if (p->pass[k].r2r_path) { p->pass[k].work = p->r2rw[0]; p->pass[k].work2 = p->r2rw[1]; }
   {
     const size_t cesz = (prec == PFK_F64 ? 16 : 8) / (is_r2r ? 2 : 1);
     for (int i = 0; i < 2; i++) {
@@ -859,7 +915,8 @@ void pa_plans_execute_batch(const pa_plan *const *plans, int count) {
         !pfk_is_device_ptr(plans[i]->out))
       uniform = 0;
     for (int k = 0; k < plans[i]->npass; k++)
-      if (plans[i]->pass[k].n1) uniform = 0;   /* long lines run in two launches */
+      if (plans[i]->pass[k].n1 || plans[i]->pass[k].r2r_path)
+        uniform = 0;   /* passes that run as several launches */
   }
   if (!uniform) {
     /* host arrays, mixed precisions or long lines: one plan after another */
@@ -943,6 +1000,7 @@ int pa_plan_npasses(const pa_plan *p) { return p ? p->npass : 0; }
 static const char *kernel_name(const pfk_pass *ps) {
   if (ps->kind == PFK_COPY) return "copy";
   if (ps->n1) return "long-two-step";
+  if (ps->r2r_path) return "r2r-inner-fft";
   const int fam = pfk_pass_kernels(ps);
   const int lm = (ps->variant & 8) && ps->kind == PFK_C2C && ps->prec == PFK_F32;
   if (fam & PFK_HAS_MFMA) {
@@ -964,7 +1022,7 @@ static const char *kernel_name(const pfk_pass *ps) {
 /* One line per pass: kind, length, strides, batch and tile geometry,
  * the kernel that runs it. */
 int pa_plan_describe(const pa_plan *p, char *buf, size_t len) {
-  static const char *kn[] = {"c2c", "r2c", "c2r", "copy"};
+  static const char *kn[] = {"c2c", "r2c", "c2r", "copy", "r2r"};   /* enum pfk_kind */
   size_t o = 0;
   buf[0] = 0;
   for (int k = 0; k < p->npass && o < len; k++) {
@@ -972,7 +1030,8 @@ int pa_plan_describe(const pa_plan *p, char *buf, size_t len) {
     o += (size_t)snprintf(buf + o, len - o,
                           "pass %d: %s n=%lld nf=%lld nin=%lld@%lld nout=%lld@%lld sin=%lld "
                           "sout=%lld tile=%lld(%lld,%lld) nb=%d roles=%d->%d kernel=%s%s\n",
-                          k, kn[ps->kind], (long long)ps->n, (long long)ps->nf,
+                          k, (ps->kind >= 0 && ps->kind <= PFK_R2R) ? kn[ps->kind] : "?",
+                          (long long)ps->n, (long long)ps->nf,
                           (long long)ps->nin, (long long)ps->zin, (long long)ps->nout,
                           (long long)ps->zout, (long long)ps->sin, (long long)ps->sout,
                           (long long)ps->text, (long long)ps->tsin, (long long)ps->tsout, ps->nb,

@@ -162,8 +162,8 @@ static void wisdom_env_save(void) {
 
 double pa_pass_bytes(const pfk_pass *ps) {
   const double rs = (ps->prec == PFK_F64) ? 8.0 : 4.0;
-  const double src_sz = (ps->kind == PFK_R2C) ? rs : 2 * rs;
-  const double dst_sz = (ps->kind == PFK_C2R) ? rs : 2 * rs;
+  const double src_sz = (ps->kind == PFK_R2C || ps->kind == PFK_R2R) ? rs : 2 * rs;
+  const double dst_sz = (ps->kind == PFK_C2R || ps->kind == PFK_R2R) ? rs : 2 * rs;
   double nl = (double)ps->text;
   for (int b = 0; b < ps->nb; b++) nl *= (double)ps->bext[b];
   return nl * ((double)ps->nin * src_sz + (double)ps->nout * dst_sz);

@@ -93,6 +93,17 @@ def test_shifted(flags):
           flags=flags)
 
 
+def test_describe_names_r2r_passes():
+    rc, comm = P.create_procmesh([1, 1])
+    d = P.DeviceBuffer(8 * 4096)
+    pl = P.Plan("r2r", [4, 8, 16], d.ptr, d.ptr, comm=comm, kinds=[P.REDFT10, P.DHT, P.RODFT00])
+    assert pl.ok
+    lines = pl.describe()
+    assert len(lines) == 3 and all(": r2r n=" in ln for ln in lines), lines
+    pl.destroy()
+    d.free()
+
+
 def test_rejections():
     rc, comm = P.create_procmesh([1, 1])
     d = P.DeviceBuffer(8 * 1024)
