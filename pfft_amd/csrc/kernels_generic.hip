@@ -202,7 +202,21 @@ static int long_c2c(const pfk_pass *p, const void *src, void *dst) {
 
 // ---- long real lines (n even, M = n/2 > PFK_MAXLINE) ----
 // Line index l (tile index innermost, then the batch indices) -> offset.
+// (32-bit divisions when the line index fits: several times cheaper)
 __device__ __forceinline__ int64_t line_offset(const pfk_pass &P, int64_t l, bool out) {
+  if (l < INT32_MAX && P.text < INT32_MAX) {
+    uint32_t u = (uint32_t)l;
+    const uint32_t t = u % (uint32_t)P.text;
+    u /= (uint32_t)P.text;
+    int64_t o = (int64_t)t * (out ? P.tsout : P.tsin);
+    for (int i = 0; i < P.nb; ++i) {
+      const uint32_t e = (uint32_t)P.bext[i];   // <= the line count
+      const uint32_t d = u % e;
+      u /= e;
+      o += (int64_t)d * (out ? P.bsout[i] : P.bsin[i]);
+    }
+    return o;
+  }
   const int64_t t = l % P.text;
   l /= P.text;
   int64_t o = t * (out ? P.tsout : P.tsin);
@@ -212,6 +226,18 @@ __device__ __forceinline__ int64_t line_offset(const pfk_pass &P, int64_t l, boo
     o += d * (out ? P.bsout[i] : P.bsin[i]);
   }
   return o;
+}
+
+// g -> (fast index, slow index) for g < fast_n * slow_n, 32-bit when it fits
+__device__ __forceinline__ void split_index(int64_t g, int64_t fast_n, int64_t *f, int64_t *sl) {
+  if (g < INT32_MAX && fast_n < INT32_MAX) {
+    const uint32_t q = (uint32_t)g / (uint32_t)fast_n;
+    *sl = q;
+    *f = (uint32_t)g - q * (uint32_t)fast_n;
+  } else {
+    *sl = g / fast_n;
+    *f = g - *sl * fast_n;
+  }
 }
 
 // r2c: Z = DFT_M(z), z_j = x_2j + i x_2j+1, lines of M in `Z`.  Output k
@@ -415,7 +441,10 @@ k_r2r_pre1(const pfk_pass P, const int64_t lines, const int64_t M, const int lm,
            const T *__restrict__ src, cpx<T> *__restrict__ w1) {
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (g >= lines * M) return;
-  const int64_t q = lm ? g % M : g / lines, l = lm ? g / M : g % lines, n = P.n;
+  int64_t q, l;
+  if (lm) split_index(g, M, &q, &l);
+  else split_index(g, lines, &l, &q);
+  const int64_t n = P.n;
   const T *s = src + line_offset(P, l, false);
   const int mode = pfk_r2r_mode(P.r2r);
   cpx<T> z;
@@ -442,7 +471,9 @@ k_r2r_post1(const pfk_pass P, const int64_t lines, const int64_t M, const int lm
             const cpx<T> *__restrict__ Z, T *__restrict__ dst) {
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (g >= lines * P.nout) return;
-  const int64_t k = lm ? g % P.nout : g / lines, l = lm ? g / P.nout : g % lines;
+  int64_t k, l;
+  if (lm) split_index(g, P.nout, &k, &l);
+  else split_index(g, lines, &l, &k);
   const int64_t n = P.n, kk = P.zout + k;
   const int mode = pfk_r2r_mode(P.r2r);
   const cpx<T> *tw2 = reinterpret_cast<const cpx<T> *>(P.tw2);
