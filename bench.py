@@ -186,6 +186,8 @@ def main():
         "config": {"workload": "%d^3 r2c(PFFT_TRANSPOSED_OUT) + c2r(PFFT_TRANSPOSED_IN), FP64, "
                                "out-of-place, procmesh 1x1" % n,
                    "n": [n, n, n], "flops_per_step": flops_pair,
+                   "input": "PFFT_PRESERVE_INPUT (default; plans never write their input, so "
+                            "PFFT_DESTROY_INPUT runs the same kernels)",
                    "alg_bytes_per_step": bytes_pair},
         "roofline": roofline,
         "pipeline": {
