@@ -190,8 +190,9 @@ extern "C" pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_grap
         if (cls) {
           int64_t lines = t->pass.text;
           for (int k = 0; k < t->pass.nb; k++) lines *= t->pass.bext[k];
-          nt = (lines + 255) / 256;
-          ld = (prec == PFK_F64) ? small::lds_bytes<double>() : small::lds_bytes<float>();
+          const int lpt = small_lines_per_tile(cls, prec == PFK_F64);
+          nt = (lines + lpt - 1) / lpt;
+          ld = small::lds_bytes(cls, prec == PFK_F64);
         } else {
           if (generic_tiling(&t->pass, &C, &pitch, &ld)) { free(b); return nullptr; }
           nt = (t->pass.text + C - 1) / C;

@@ -1,5 +1,6 @@
 // kernels_small.hip -- per-pass launch of the register-resident short-length
-// c2c kernel (small_tile.hpp): one line per lane, 256 lines per workgroup.
+// c2c kernel (small_tile.hpp): one line per lane (256 lines per workgroup)
+// or G lanes per line (256/G lines) for the longer lengths.
 #include "small_tile.hpp"
 
 namespace pfk {
@@ -25,12 +26,14 @@ static int launch_small_nf(const pfk_pass *p, const void *src, void *dst) {
   } else {
     int64_t nlines = p->text;
     for (int i = 0; i < p->nb; ++i) nlines *= p->bext[i];
-    const int64_t ntiles = (nlines + 255) / 256;
+    constexpr bool F64 = sizeof(T) == 8;
+    constexpr int lpt = small_lines_per_tile(NF, F64);
+    const int64_t ntiles = (nlines + lpt - 1) / lpt;
     const int64_t chunk = (int64_t)1 << 30;
     for (int64_t base = 0; base < ntiles; base += chunk) {
       const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
       hipLaunchKernelGGL((k_small<T, NF>), dim3((unsigned)nblk), dim3(256),
-                         small::lds_bytes<T>(), g_stream, *p, base, (const T *)src, (T *)dst);
+                         small::lds_bytes(NF, F64), g_stream, *p, base, (const T *)src, (T *)dst);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) { set_error("pfk_launch_pass(small)", e); return -1; }
     }

@@ -129,7 +129,8 @@ __device__ __forceinline__ void dft(C2<T> *a, T sg) {
 }
 
 // Stages S.. of the register FFT; L = product of the radices before S.
-template <typename T, int NF, int S, int L>
+// tw[TS*m] = exp(sg*2*pi*i*m/NF) (a table of a TS times longer length).
+template <typename T, int NF, int S, int L, int TS = 1>
 __device__ __forceinline__ void fft(C2<T> *x, const C2<T> *__restrict__ tw, T sg) {
   if constexpr (S < nstages(NF)) {
     constexpr int R = radix_at(NF, S);
@@ -142,7 +143,7 @@ __device__ __forceinline__ void fft(C2<T> *x, const C2<T> *__restrict__ tw, T sg
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         a[r] = x[t + r * NR];
-        if (k != 0 && r != 0) a[r] = cmul(a[r], tw[r * k * M]);
+        if (k != 0 && r != 0) a[r] = cmul(a[r], tw[r * k * M * TS]);
       }
       dft<T, R>(a, sg);
 #pragma unroll
@@ -150,7 +151,7 @@ __device__ __forceinline__ void fft(C2<T> *x, const C2<T> *__restrict__ tw, T sg
     }
 #pragma unroll
     for (int i = 0; i < NF; ++i) x[i] = y[i];
-    fft<T, NF, S + 1, LR>(x, tw, sg);
+    fft<T, NF, S + 1, LR, TS>(x, tw, sg);
   }
 }
 
@@ -167,19 +168,71 @@ __device__ __forceinline__ T modsign(const pfk_mod &m, int64_t j) {
 template <typename T>
 constexpr int chunk() { return 128 / (int)sizeof(C2<T>); }
 
-// LDS bytes one workgroup needs
-template <typename T>
-constexpr size_t lds_bytes() {
-  return 4 * 64 * (chunk<T>() + 1) * sizeof(C2<T>) + 2 * 256 * sizeof(int64_t);
+// Shape of the split form (G lanes per line, see small_split_tile).
+struct split_shape {
+  int g;     // lanes per line (1: one line per lane)
+  int m;     // elements per lane = NF / g
+  int ni;    // k1 groups per lane in the second step = ceil(m / g)
+  int lw;    // lines per wave = 64 / g
+  int lpt;   // lines per workgroup tile = 256 / g
+  int ib;    // k1 groups per LDS chunk (divides ni)
+  int wc;    // chunk width in complex slots = ib * g * g
+  int nch;   // chunks per line = ni / ib (nch * wc >= NF)
+  int rp;    // LDS row pitch = wc + 1
+};
+
+// G: the smallest of 2, 4, 8 dividing NF with NF/G <= 16 (failing that the
+// largest dividing one); 1 for NF <= 16.  When G divides M the line is
+// staged and exchanged in chunks of at most 8 KiB per wave (16 complex64
+// or 8 complex128 slots per lane); otherwise in one chunk of ni*G*G >= NF
+// slots, the slots of k1 >= M being unused.
+__host__ __device__ constexpr split_shape split_of(int nf, bool f64) {
+  int g = 1;
+  if (nf > 16) {
+    for (int c = 8; c >= 2; c /= 2)
+      if (nf % c == 0 && g == 1) g = c;   // largest divisor as the fallback
+    for (int c = 2; c <= 8; c *= 2)
+      if (nf % c == 0 && nf / c <= 16) { g = c; break; }
+  }
+  split_shape s{};
+  s.g = g;
+  s.m = nf / g;
+  s.ni = (s.m + g - 1) / g;
+  s.lw = 64 / g;
+  s.lpt = 256 / g;
+  s.ib = s.ni;
+  if (s.m % g == 0) {
+    const int wmax = (f64 ? 8 : 16) * g;
+    s.ib = 1;
+    for (int d = 1; d <= s.ni; ++d)
+      if (s.ni % d == 0 && d * g * g <= wmax) s.ib = d;
+  }
+  s.wc = s.ib * g * g;
+  s.nch = s.ni / s.ib;
+  s.rp = s.wc + 1;
+  return s;
+}
+
+// LDS bytes one workgroup needs for length nf
+__host__ __device__ constexpr size_t lds_bytes(int nf, bool f64) {
+  const split_shape s = split_of(nf, f64);
+  const size_t csz = f64 ? 16 : 8;
+  if (s.g == 1) return 4 * 64 * (size_t)(128 / csz + 1) * csz + 2 * 256 * sizeof(int64_t);
+  return 4 * (size_t)s.lw * s.rp * csz + 2 * (size_t)s.lpt * sizeof(int64_t);
 }
 
 }  // namespace small
 
-// One tile (256 lines) of a c2c pass of length NF (= P.n).
+// lines one workgroup tile of the short-length kernel covers
+__host__ __device__ constexpr int small_lines_per_tile(int nf, bool f64) {
+  return small::split_of(nf, f64).lpt;
+}
+
+// One tile (256 lines, one per lane) of a c2c pass of length NF (= P.n).
 template <typename T, int NF>
-__device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile,
-                                           const T *__restrict__ src, T *__restrict__ dst,
-                                           unsigned char *smem) {
+__device__ __forceinline__ void small_line_tile(const pfk_pass &P, const int64_t tile,
+                                                const T *__restrict__ src, T *__restrict__ dst,
+                                                unsigned char *smem) {
   using namespace small;
   using CT = C2<T>;
   constexpr int K = chunk<T>();
@@ -306,6 +359,207 @@ __device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile
       }
     }
   }
+}
+
+// One tile (256/G lines) of a c2c pass of length NF = G*M, G lanes per
+// line.  Lane j (0 <= j < G) of a line holds the decimated sequence
+// x[j + G*m], m in [0, M), and:
+//   1. takes its M-point DFT in registers, Y_j[k1] (k1 in [0, M));
+//   2. exchanges through LDS so that lane j' holds, for k1 = j' + G*i
+//      (i in [0, ceil(M/G))), the G values Y_0[k1] .. Y_{G-1}[k1], and
+//      multiplies Y_jj[k1] by W_NF^(jj*k1) = W_NF^(jj*j') W_M^(jj*i);
+//   3. takes the G-point DFTs over jj: X[k1 + M*k2] = sum_jj W_G^(jj*k2) (..).
+// This is the two-factor split NF = G*M of the DFT (n = j + G*m,
+// k = k1 + M*k2) with the first factor over lanes.  Slots with k1 >= M
+// (when G does not divide M) are computed and dropped.  The lanes of a wave
+// take 64/G lines, line index fastest: lane = l + (64/G)*j, so that with
+// lines adjacent along the tile dimension each load or store instruction
+// covers G runs of 64/G adjacent lines.  Contiguous lines are staged through
+// LDS in chunks of wc columns, as the one-line-per-lane form does; the LDS
+// chunk of a wave is also the exchange buffer of step 2, done in the same
+// number of chunks.
+template <typename T, int NF>
+__device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_t tile,
+                                                 const T *__restrict__ src, T *__restrict__ dst,
+                                                 unsigned char *smem) {
+  using namespace small;
+  using CT = C2<T>;
+  constexpr split_shape SH = split_of(NF, sizeof(T) == 8);
+  constexpr int G = SH.g, M = SH.m, NI = SH.ni, LW = SH.lw, LPT = SH.lpt, IB = SH.ib,
+                WC = SH.wc, NCH = SH.nch, RP = SH.rp;
+  constexpr int Q = LW * WC / 64;   // staged slots per lane and chunk (= IB*G)
+  constexpr bool EXACT = M % G == 0;
+  static_assert(G > 1 && G * M == NF && NI * G >= M && NCH * WC >= NF && Q == IB * G, "shape");
+  static_assert(EXACT || NCH == 1, "uneven split in one chunk");
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l = lane % LW, j = lane / LW;   // line within the wave, lane within the line
+  const int row0 = wave * LW;               // first line of the wave within the tile
+  int64_t nlines = P.text;
+  for (int i = 0; i < P.nb; ++i) nlines *= P.bext[i];
+  CT *buf = reinterpret_cast<CT *>(smem) + wave * LW * RP;
+  int64_t *sbase = reinterpret_cast<int64_t *>(smem + 4 * LW * RP * sizeof(CT));
+  int64_t *dbase = sbase + LPT;
+  if (tid < LPT) {
+    const int64_t line = tile * LPT + tid;
+    int64_t sb = -1, db = -1;
+    if (line < nlines) {
+      const int64_t c = line % P.text;
+      int64_t bflat = line / P.text;
+      sb = c * P.tsin;
+      db = c * P.tsout;
+      for (int i = 0; i < P.nb; ++i) {
+        const int64_t e = P.bext[i];
+        const int64_t d = bflat % e;
+        bflat /= e;
+        sb += d * P.bsin[i];
+        db += d * P.bsout[i];
+      }
+    }
+    sbase[tid] = sb;
+    dbase[tid] = db;
+  }
+  lds_barrier();
+  const int64_t sb = sbase[row0 + l], db = dbase[row0 + l];
+  const bool active = sb >= 0;
+  const CT *s = reinterpret_cast<const CT *>(src);
+  CT *d = reinterpret_cast<CT *>(dst);
+  const int nin = (int)P.nin, zin = (int)P.zin, nout = (int)P.nout, zout = (int)P.zout;
+  const bool lds_in = P.sin == 1 && (P.tsin != 1 || P.text == 1);
+  const bool lds_out = P.sout == 1 && (P.tsout != 1 || P.text == 1);
+
+  // ---------------- load: x[m] = line[j + G*m] (embed + input modulation) ----------------
+  CT x[NI * G];
+  if (lds_in) {
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      CT v[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {   // issue every load of the chunk first
+        const int e = lane + 64 * q;
+        const int row = e / WC, col = e - row * WC;
+        const int64_t rb = sbase[row0 + row];
+        const int jj = c * WC + col - zin;
+        const bool ok = rb >= 0 && c * WC + col < NF && jj >= 0 && jj < nin;
+        v[q] = s[ok ? rb + jj : 0];
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int e = lane + 64 * q;
+        const int row = e / WC, col = e - row * WC;
+        const int64_t rb = sbase[row0 + row];
+        const int jj = c * WC + col - zin;
+        const bool ok = rb >= 0 && c * WC + col < NF && jj >= 0 && jj < nin;
+        const T f = modsign<T>(P.imod, jj);
+        buf[row * RP + col] = ok ? CT{v[q].x * f, v[q].y * f} : CT{(T)0, (T)0};
+      }
+      lds_barrier();
+#pragma unroll
+      for (int mm = 0; mm < IB * G; ++mm)
+        if (c * IB * G + mm < M) x[c * IB * G + mm] = buf[l * RP + j + G * mm];
+      lds_barrier();
+    }
+  } else {
+    const int64_t sbv = active ? sb : 0;   // inactive lanes read element 0 and drop it
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int jj = j + G * m - zin;
+      const bool ok = jj >= 0 && jj < nin;
+      x[m] = s[sbv + (int64_t)(ok ? jj : 0) * P.sin];
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int jj = j + G * m - zin;
+      const bool ok = active && jj >= 0 && jj < nin;
+      const T f = modsign<T>(P.imod, jj);
+      x[m] = ok ? CT{x[m].x * f, x[m].y * f} : CT{(T)0, (T)0};
+    }
+  }
+
+  // ---------------- step 1: M-point DFT of the lane's sequence ----------------
+  const T sg = (T)P.sign;
+  const CT *tw = reinterpret_cast<const CT *>(P.tw);
+  fft<T, M, 0, 1, G>(x, tw, sg);
+
+  // ---------------- exchange: x[i*G + jj] = Y_jj[j + G*i] ----------------
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+    for (int kk = 0; kk < IB * G; ++kk)
+      if (c * IB * G + kk < M) buf[l * RP + kk * G + j] = x[c * IB * G + kk];
+    lds_barrier();
+#pragma unroll
+    for (int ii = 0; ii < IB; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < G; ++jj)
+        x[(c * IB + ii) * G + jj] = buf[l * RP + (j + G * ii) * G + jj];
+    lds_barrier();
+  }
+
+  // ---------------- twiddle, step 2: x[i*G + k2] = X[j + G*i + M*k2] ----------------
+  CT u[G];
+#pragma unroll
+  for (int jj = 1; jj < G; ++jj) u[jj] = tw[jj * j];   // W_NF^(jj*j)
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+#pragma unroll
+    for (int jj = 1; jj < G; ++jj) {
+      CT t = cmul(x[i * G + jj], u[jj]);
+      if (i != 0) t = cmul(t, tw[G * ((jj * i) % M)]);   // W_M^(jj*i)
+      x[i * G + jj] = t;
+    }
+    dft<T, G>(x + i * G, sg);
+  }
+
+  // ---------------- store (trunc + output modulation) ----------------
+  if (lds_out) {
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+      for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int k2 = 0; k2 < G; ++k2) {
+          const int base = G * i + M * k2;   // column of lane 0; lane j adds j
+          const bool mine = EXACT ? (base >= c * WC && base < (c + 1) * WC) : (j + G * i < M);
+          if (mine) buf[l * RP + base - c * WC + j] = x[i * G + k2];
+        }
+      lds_barrier();
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int e = lane + 64 * q;
+        const int row = e / WC, col = e - row * WC;
+        const int64_t rb = dbase[row0 + row];
+        const int kk = c * WC + col - zout;
+        if (rb >= 0 && c * WC + col < NF && kk >= 0 && kk < nout) {
+          const CT v = buf[row * RP + col];
+          const T f = modsign<T>(P.omod, kk);
+          d[rb + kk] = CT{v.x * f, v.y * f};
+        }
+      }
+      lds_barrier();
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+      for (int k2 = 0; k2 < G; ++k2) {
+        const int kk = j + G * i + M * k2 - zout;
+        if (active && (EXACT || j + G * i < M) && kk >= 0 && kk < nout) {
+          const T f = modsign<T>(P.omod, kk);
+          d[db + (int64_t)kk * P.sout] = CT{x[i * G + k2].x * f, x[i * G + k2].y * f};
+        }
+      }
+  }
+}
+
+// One tile of a c2c pass of length NF: small_lines_per_tile(NF) lines.
+template <typename T, int NF>
+__device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile,
+                                           const T *__restrict__ src, T *__restrict__ dst,
+                                           unsigned char *smem) {
+  if constexpr (small::split_of(NF, sizeof(T) == 8).g > 1)
+    small_split_tile<T, NF>(P, tile, src, dst, smem);
+  else
+    small_line_tile<T, NF>(P, tile, src, dst, smem);
 }
 
 // Lengths with a small_tile instantiation, per precision.
