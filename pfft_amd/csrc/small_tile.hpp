@@ -1,15 +1,16 @@
 // small_tile.hpp -- register-resident c2c pass for short mixed-radix
-// lengths (NF = 2^a 3^b 5^c <= 128 complex64, <= 64 complex128), shared by
-// the per-pass launch (kernels_small.hip) and the persistent batch kernel
-// (kernels_batch.hip).
+// lengths (NF = 2^a 3^b 5^c <= 128), shared by the per-pass launch
+// (kernels_small.hip) and the persistent batch kernel (kernels_batch.hip).
 //
-// Work split.  A workgroup of 256 threads transforms 256 lines, one line
-// per lane: the lines of the pass are numbered flat over the tile
-// dimension and the batch dimensions, and lane `tid` of tile `b` takes line
-// 256*b + tid.  The whole line lives in registers, so the FFT stages need
-// no LDS and no barriers; every index and every twiddle index is a
-// compile-time constant after unrolling (the twiddles are wave-uniform
-// loads from the plan's exp(sign*2*pi*i*m/NF) table).
+// Work split.  For NF <= 16 a workgroup of 256 threads transforms 256
+// lines, one line per lane: the lines of the pass are numbered flat over
+// the tile dimension and the batch dimensions, and lane `tid` of tile `b`
+// takes line 256*b + tid.  The whole line lives in registers, so the FFT
+// stages need no LDS and no barriers; every index and every twiddle index
+// is a compile-time constant after unrolling (the twiddles are wave-uniform
+// loads from the plan's exp(sign*2*pi*i*m/NF) table).  Longer lines are
+// split over G lanes (small_split_tile), so that a lane holds at most 16
+// elements and the kernel keeps several waves per SIMD.
 //
 // Coalescing.  When the lines are adjacent along the tile dimension
 // (tile stride 1) lanes read element j of 64 consecutive lines together:
@@ -178,7 +179,8 @@ struct split_shape {
   int ib;    // k1 groups per LDS chunk (divides ni)
   int wc;    // chunk width in complex slots = ib * g * g
   int nch;   // chunks per line = ni / ib (nch * wc >= NF)
-  int rp;    // LDS row pitch = wc + 1
+  int xs;    // exchange slots per k1
+  int rp;    // LDS row pitch
 };
 
 // G: the smallest of 2, 4, 8 dividing NF with NF/G <= 16 (failing that the
@@ -186,13 +188,22 @@ struct split_shape {
 // staged and exchanged in chunks of at most 8 KiB per wave (16 complex64
 // or 8 complex128 slots per lane); otherwise in one chunk of ni*G*G >= NF
 // slots, the slots of k1 >= M being unused.
+#ifndef PFK_SPLIT_MAXM
+#define PFK_SPLIT_MAXM 16
+#endif
+#ifndef PFK_SPLIT_RPAD8
+#define PFK_SPLIT_RPAD8 1
+#endif
+#ifndef PFK_SPLIT_XPAD
+#define PFK_SPLIT_XPAD 0
+#endif
 __host__ __device__ constexpr split_shape split_of(int nf, bool f64) {
   int g = 1;
   if (nf > 16) {
     for (int c = 8; c >= 2; c /= 2)
       if (nf % c == 0 && g == 1) g = c;   // largest divisor as the fallback
     for (int c = 2; c <= 8; c *= 2)
-      if (nf % c == 0 && nf / c <= 16) { g = c; break; }
+      if (nf % c == 0 && nf / c <= PFK_SPLIT_MAXM) { g = c; break; }
   }
   split_shape s{};
   s.g = g;
@@ -209,7 +220,8 @@ __host__ __device__ constexpr split_shape split_of(int nf, bool f64) {
   }
   s.wc = s.ib * g * g;
   s.nch = s.ni / s.ib;
-  s.rp = s.wc + 1;
+  s.xs = g + PFK_SPLIT_XPAD;
+  s.rp = (s.wc > s.ib * g * s.xs ? s.wc : s.ib * g * s.xs) + (g == 8 ? PFK_SPLIT_RPAD8 : 1);
   return s;
 }
 
@@ -386,7 +398,7 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
   using CT = C2<T>;
   constexpr split_shape SH = split_of(NF, sizeof(T) == 8);
   constexpr int G = SH.g, M = SH.m, NI = SH.ni, LW = SH.lw, LPT = SH.lpt, IB = SH.ib,
-                WC = SH.wc, NCH = SH.nch, RP = SH.rp;
+                WC = SH.wc, NCH = SH.nch, RP = SH.rp, XS = SH.xs;
   constexpr int Q = LW * WC / 64;   // staged slots per lane and chunk (= IB*G)
   constexpr bool EXACT = M % G == 0;
   static_assert(G > 1 && G * M == NF && NI * G >= M && NCH * WC >= NF && Q == IB * G, "shape");
@@ -485,13 +497,13 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
   for (int c = 0; c < NCH; ++c) {
 #pragma unroll
     for (int kk = 0; kk < IB * G; ++kk)
-      if (c * IB * G + kk < M) buf[l * RP + kk * G + j] = x[c * IB * G + kk];
+      if (c * IB * G + kk < M) buf[l * RP + kk * XS + j] = x[c * IB * G + kk];
     lds_barrier();
 #pragma unroll
     for (int ii = 0; ii < IB; ++ii)
 #pragma unroll
       for (int jj = 0; jj < G; ++jj)
-        x[(c * IB + ii) * G + jj] = buf[l * RP + (j + G * ii) * G + jj];
+        x[(c * IB + ii) * G + jj] = buf[l * RP + (j + G * ii) * XS + jj];
     lds_barrier();
   }
 
@@ -562,14 +574,14 @@ __device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile
     small_line_tile<T, NF>(P, tile, src, dst, smem);
 }
 
-// Lengths with a small_tile instantiation, per precision.
+// Lengths with a small_tile instantiation (the same in both precisions).
 __host__ __device__ constexpr bool small_length(int nf, bool f64) {
+  (void)f64;
   switch (nf) {
     case 2: case 3: case 4: case 5: case 6: case 8: case 10: case 12: case 15: case 16:
     case 20: case 24: case 30: case 32: case 40: case 48: case 60: case 64:
-      return true;
     case 80: case 96: case 120: case 128:
-      return !f64;
+      return true;
     default:
       return false;
   }

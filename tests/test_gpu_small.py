@@ -17,7 +17,6 @@ pytestmark = pytest.mark.gpu
 
 SI, SO, TO = P.SHIFTED_IN, P.SHIFTED_OUT, P.TRANSPOSED_OUT
 LENGTHS = [2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20, 24, 30, 32, 40, 48, 60, 64, 80, 96, 120, 128]
-F64_MAX = 64
 
 
 def tol(prec):
@@ -45,8 +44,6 @@ def run_batch(spec, x, prec):
 def cases():
     for L in LENGTHS:
         for prec in ("f32", "f64"):
-            if prec == "f64" and L > F64_MAX:
-                continue
             yield L, prec
 
 
@@ -68,8 +65,6 @@ def test_small_lengths(L, prec):
 @pytest.mark.parametrize("L", [24, 48, 60, 80, 96, 120, 128])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_small_embed_trunc_shift(L, prec):
-    if prec == "f64" and L > F64_MAX:
-        pytest.skip("no FP64 short-length kernel for this length")
     # shifts need even n, ni and no in every dimension
     for ni, no, flags in (([L - 5, 3, L - 7], [L - 3, 3, L - 9], 0),
                           ([L - 4, 4, L - 6], [L - 2, 4, L - 8], SI | SO),

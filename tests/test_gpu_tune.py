@@ -68,7 +68,7 @@ def _expected(line, prec, variant, remap):
     m = re.search(r"pass \d+: (\w+) n=(\d+) nf=(\d+)", line)
     kind, n, nf = m.group(1), int(m.group(2)), int(m.group(3))
     pow2 = 64 <= nf <= 4096 and nf & (nf - 1) == 0
-    small = kind == "c2c" and nf in SMALL and (prec == "f32" or nf <= 64)
+    small = kind == "c2c" and nf in SMALL
     if variant == 1:
         return "general"
     if variant == 3 and small:
@@ -112,8 +112,8 @@ def test_measure_fills_cache_and_reuses(tune_all):
     _plan_lines("c2c", [64, 128, 256], TO, "f64")
     n1, t1 = P.wisdom_count(), P.tune_timings()
     # three passes; candidates per pass: eight power-of-two variants, the
-    # short-length kernel where the length has one (64 in FP64), the general one
-    assert n1 == 3 and t1 - t0 == 10 + 9 + 9
+    # short-length kernel where the length has one (64 and 128), the general one
+    assert n1 == 3 and t1 - t0 == 10 + 10 + 9
     _plan_lines("c2c", [64, 128, 256], TO, "f64")
     assert P.wisdom_count() == n1 and P.tune_timings() == t1
     # different geometry -> new selections
