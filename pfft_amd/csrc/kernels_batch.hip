@@ -57,6 +57,7 @@ template <typename T, int NF>
 __global__ void __launch_bounds__(256)
 k_batch_small(const pfk_task *__restrict__ tasks, const int t0, const int t1,
               const int64_t item0, const int64_t item1) {
+  static_assert(small_threads(NF, sizeof(T) == 8) == 256, "batch launches run 256 threads");
   extern __shared__ unsigned char smem_raw[];
   for (int64_t g = item0 + blockIdx.x; g < item1; g += gridDim.x) {
     const pfk_task &tk = tasks[find_task(tasks, t0, t1, g)];
@@ -126,7 +127,9 @@ static int batch_launch(pfk_batch *b) {
 }
 
 static int task_class(const pfk_task *t) {
-  if (!g_generic_only && t->pass.variant != 1 && small_covers(&t->pass)) return (int)t->pass.nf;
+  if (!g_generic_only && t->pass.variant != 1 && small_covers(&t->pass) &&
+      small_length((int)t->pass.nf, t->pass.prec == PFK_F64))
+    return (int)t->pass.nf;
   return 0;
 }
 

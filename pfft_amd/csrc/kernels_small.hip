@@ -1,6 +1,7 @@
 // kernels_small.hip -- per-pass launch of the register-resident short-length
 // c2c kernel (small_tile.hpp): one line per lane (256 lines per workgroup)
-// or G lanes per line (256/G lines) for the longer lengths.
+// or G lanes per line (256/G lines) for the longer lengths, including the
+// mid lengths 144..960 that only this per-pass launch instantiates.
 #include "small_tile.hpp"
 
 namespace pfk {
@@ -15,13 +16,16 @@ k_small(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
 
 int small_covers(const pfk_pass *p) {
   if (p->n1 || p->dec_in || p->dec_out || p->twn) return 0;   // steps of long lines
-  if (p->kind != PFK_C2C || p->hw || p->nf != p->n || p->n > 128) return 0;
-  return small_length((int)p->nf, p->prec == PFK_F64) ? 1 : 0;
+  if (p->kind != PFK_C2C || p->hw || p->nf != p->n || p->nf > 4096) return 0;
+  const int nf = (int)p->nf;
+  const bool f64 = p->prec == PFK_F64;
+  if (small_length(nf, f64)) return 1;
+  return (mid_length(nf, f64) && small::lds_bytes(nf, f64) <= (size_t)g_max_lds) ? 1 : 0;
 }
 
 template <typename T, int NF>
 static int launch_small_nf(const pfk_pass *p, const void *src, void *dst) {
-  if constexpr (!small_length(NF, sizeof(T) == 8)) {
+  if constexpr (!small_length(NF, sizeof(T) == 8) && !mid_length(NF, sizeof(T) == 8)) {
     return 1;
   } else {
     int64_t nlines = p->text;
@@ -32,7 +36,7 @@ static int launch_small_nf(const pfk_pass *p, const void *src, void *dst) {
     const int64_t chunk = (int64_t)1 << 30;
     for (int64_t base = 0; base < ntiles; base += chunk) {
       const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
-      hipLaunchKernelGGL((k_small<T, NF>), dim3((unsigned)nblk), dim3(256),
+      hipLaunchKernelGGL((k_small<T, NF>), dim3((unsigned)nblk), dim3(small_threads(NF, F64)),
                          small::lds_bytes(NF, F64), g_stream, *p, base, (const T *)src, (T *)dst);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) { set_error("pfk_launch_pass(small)", e); return -1; }
@@ -51,6 +55,11 @@ static int launch_small_t(const pfk_pass *p, const void *src, void *dst) {
     PFK_SMALL_LAUNCH(30) PFK_SMALL_LAUNCH(32) PFK_SMALL_LAUNCH(40) PFK_SMALL_LAUNCH(48)
     PFK_SMALL_LAUNCH(60) PFK_SMALL_LAUNCH(64) PFK_SMALL_LAUNCH(80) PFK_SMALL_LAUNCH(96)
     PFK_SMALL_LAUNCH(120) PFK_SMALL_LAUNCH(128)
+    PFK_SMALL_LAUNCH(144) PFK_SMALL_LAUNCH(160) PFK_SMALL_LAUNCH(192) PFK_SMALL_LAUNCH(200)
+    PFK_SMALL_LAUNCH(216) PFK_SMALL_LAUNCH(240) PFK_SMALL_LAUNCH(288) PFK_SMALL_LAUNCH(320)
+    PFK_SMALL_LAUNCH(384) PFK_SMALL_LAUNCH(400) PFK_SMALL_LAUNCH(432) PFK_SMALL_LAUNCH(480)
+    PFK_SMALL_LAUNCH(576) PFK_SMALL_LAUNCH(640) PFK_SMALL_LAUNCH(768) PFK_SMALL_LAUNCH(800)
+    PFK_SMALL_LAUNCH(864) PFK_SMALL_LAUNCH(960)
 #undef PFK_SMALL_LAUNCH
     default: return 1;
   }

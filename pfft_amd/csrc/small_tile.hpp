@@ -175,53 +175,58 @@ struct split_shape {
   int m;     // elements per lane = NF / g
   int ni;    // k1 groups per lane in the second step = ceil(m / g)
   int lw;    // lines per wave = 64 / g
-  int lpt;   // lines per workgroup tile = 256 / g
+  int nw;    // waves per workgroup (4; fewer where 4 waves' LDS exceeds 64 KiB)
+  int lpt;   // lines per workgroup tile = nw * lw
   int ib;    // k1 groups per LDS chunk (divides ni)
-  int wc;    // chunk width in complex slots = ib * g * g
-  int nch;   // chunks per line = ni / ib (nch * wc >= NF)
-  int xs;    // exchange slots per k1
-  int rp;    // LDS row pitch
+  int wc;    // chunk width in complex slots (ib * g * g, or NF in one chunk)
+  int nch;   // chunks per line (nch * wc = NF)
+  int rp;    // LDS row pitch = wc + 1
 };
 
-// G: the smallest of 2, 4, 8 dividing NF with NF/G <= 16 (failing that the
-// largest dividing one); 1 for NF <= 16.  When G divides M the line is
+// G for NF <= 128: the smallest of 2, 4, 8 dividing NF with NF/G <= 16
+// (failing that the largest dividing one); 1 for NF <= 16.  For NF > 128:
+// the power of two up to 64 dividing NF that minimises the larger of the
+// two per-lane counts, M and ceil(M/G)*G.  When G divides M the line is
 // staged and exchanged in chunks of at most 8 KiB per wave (16 complex64
-// or 8 complex128 slots per lane); otherwise in one chunk of ni*G*G >= NF
-// slots, the slots of k1 >= M being unused.
-#ifndef PFK_SPLIT_MAXM
-#define PFK_SPLIT_MAXM 16
-#endif
-#ifndef PFK_SPLIT_RPAD8
-#define PFK_SPLIT_RPAD8 1
-#endif
-#ifndef PFK_SPLIT_XPAD
-#define PFK_SPLIT_XPAD 0
-#endif
+// or 8 complex128 slots per lane); otherwise in one chunk of NF slots.
 __host__ __device__ constexpr split_shape split_of(int nf, bool f64) {
   int g = 1;
-  if (nf > 16) {
+  if (nf > 128) {
+    int best = 1 << 30;
+    for (int c = 2; c <= 64; c *= 2) {
+      if (nf % c != 0) continue;
+      const int m = nf / c, held = ((m + c - 1) / c) * c;
+      const int cost = m > held ? m : held;
+      if (cost < best) { best = cost; g = c; }
+    }
+  } else if (nf > 16) {
     for (int c = 8; c >= 2; c /= 2)
       if (nf % c == 0 && g == 1) g = c;   // largest divisor as the fallback
     for (int c = 2; c <= 8; c *= 2)
-      if (nf % c == 0 && nf / c <= PFK_SPLIT_MAXM) { g = c; break; }
+      if (nf % c == 0 && nf / c <= 16) { g = c; break; }
   }
   split_shape s{};
   s.g = g;
   s.m = nf / g;
   s.ni = (s.m + g - 1) / g;
   s.lw = 64 / g;
-  s.lpt = 256 / g;
-  s.ib = s.ni;
   if (s.m % g == 0) {
     const int wmax = (f64 ? 8 : 16) * g;
     s.ib = 1;
     for (int d = 1; d <= s.ni; ++d)
       if (s.ni % d == 0 && d * g * g <= wmax) s.ib = d;
+    s.wc = s.ib * g * g;
+    s.nch = s.ni / s.ib;
+  } else {
+    s.ib = s.ni;
+    s.wc = nf;
+    s.nch = 1;
   }
-  s.wc = s.ib * g * g;
-  s.nch = s.ni / s.ib;
-  s.xs = g + PFK_SPLIT_XPAD;
-  s.rp = (s.wc > s.ib * g * s.xs ? s.wc : s.ib * g * s.xs) + (g == 8 ? PFK_SPLIT_RPAD8 : 1);
+  s.rp = s.wc + 1;
+  const int csz = f64 ? 16 : 8;
+  s.nw = 4;
+  while (s.nw > 1 && s.nw * s.lw * s.rp * csz + 2 * s.nw * s.lw * 8 > 65536) s.nw /= 2;
+  s.lpt = s.nw * s.lw;
   return s;
 }
 
@@ -230,7 +235,7 @@ __host__ __device__ constexpr size_t lds_bytes(int nf, bool f64) {
   const split_shape s = split_of(nf, f64);
   const size_t csz = f64 ? 16 : 8;
   if (s.g == 1) return 4 * 64 * (size_t)(128 / csz + 1) * csz + 2 * 256 * sizeof(int64_t);
-  return 4 * (size_t)s.lw * s.rp * csz + 2 * (size_t)s.lpt * sizeof(int64_t);
+  return (size_t)s.nw * s.lw * s.rp * csz + 2 * (size_t)s.lpt * sizeof(int64_t);
 }
 
 }  // namespace small
@@ -238,6 +243,11 @@ __host__ __device__ constexpr size_t lds_bytes(int nf, bool f64) {
 // lines one workgroup tile of the short-length kernel covers
 __host__ __device__ constexpr int small_lines_per_tile(int nf, bool f64) {
   return small::split_of(nf, f64).lpt;
+}
+
+// threads per workgroup of the short-length kernel
+__host__ __device__ constexpr int small_threads(int nf, bool f64) {
+  return 64 * small::split_of(nf, f64).nw;
 }
 
 // One tile (256 lines, one per lane) of a c2c pass of length NF (= P.n).
@@ -373,7 +383,7 @@ __device__ __forceinline__ void small_line_tile(const pfk_pass &P, const int64_t
   }
 }
 
-// One tile (256/G lines) of a c2c pass of length NF = G*M, G lanes per
+// One tile (NW*64/G lines) of a c2c pass of length NF = G*M, G lanes per
 // line.  Lane j (0 <= j < G) of a line holds the decimated sequence
 // x[j + G*m], m in [0, M), and:
 //   1. takes its M-point DFT in registers, Y_j[k1] (k1 in [0, M));
@@ -397,11 +407,13 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
   using namespace small;
   using CT = C2<T>;
   constexpr split_shape SH = split_of(NF, sizeof(T) == 8);
-  constexpr int G = SH.g, M = SH.m, NI = SH.ni, LW = SH.lw, LPT = SH.lpt, IB = SH.ib,
-                WC = SH.wc, NCH = SH.nch, RP = SH.rp, XS = SH.xs;
-  constexpr int Q = LW * WC / 64;   // staged slots per lane and chunk (= IB*G)
+  constexpr int G = SH.g, M = SH.m, NI = SH.ni, LW = SH.lw, LPT = SH.lpt, NW = SH.nw, IB = SH.ib,
+                WC = SH.wc, NCH = SH.nch, RP = SH.rp;
   constexpr bool EXACT = M % G == 0;
-  static_assert(G > 1 && G * M == NF && NI * G >= M && NCH * WC >= NF && Q == IB * G, "shape");
+  constexpr int Q = LW * WC / 64;   // staged slots per lane and chunk (IB*G, or M)
+  constexpr int QL = EXACT ? IB * G : M;
+  static_assert(G > 1 && G * M == NF && NI * G >= M && NCH * WC == NF && Q * 64 == LW * WC &&
+                Q == QL, "shape");
   static_assert(EXACT || NCH == 1, "uneven split in one chunk");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l = lane % LW, j = lane / LW;   // line within the wave, lane within the line
@@ -409,7 +421,7 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
   int64_t nlines = P.text;
   for (int i = 0; i < P.nb; ++i) nlines *= P.bext[i];
   CT *buf = reinterpret_cast<CT *>(smem) + wave * LW * RP;
-  int64_t *sbase = reinterpret_cast<int64_t *>(smem + 4 * LW * RP * sizeof(CT));
+  int64_t *sbase = reinterpret_cast<int64_t *>(smem + NW * LW * RP * sizeof(CT));
   int64_t *dbase = sbase + LPT;
   if (tid < LPT) {
     const int64_t line = tile * LPT + tid;
@@ -451,7 +463,7 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
         const int row = e / WC, col = e - row * WC;
         const int64_t rb = sbase[row0 + row];
         const int jj = c * WC + col - zin;
-        const bool ok = rb >= 0 && c * WC + col < NF && jj >= 0 && jj < nin;
+        const bool ok = rb >= 0 && jj >= 0 && jj < nin;
         v[q] = s[ok ? rb + jj : 0];
       }
 #pragma unroll
@@ -460,14 +472,13 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
         const int row = e / WC, col = e - row * WC;
         const int64_t rb = sbase[row0 + row];
         const int jj = c * WC + col - zin;
-        const bool ok = rb >= 0 && c * WC + col < NF && jj >= 0 && jj < nin;
+        const bool ok = rb >= 0 && jj >= 0 && jj < nin;
         const T f = modsign<T>(P.imod, jj);
         buf[row * RP + col] = ok ? CT{v[q].x * f, v[q].y * f} : CT{(T)0, (T)0};
       }
       lds_barrier();
 #pragma unroll
-      for (int mm = 0; mm < IB * G; ++mm)
-        if (c * IB * G + mm < M) x[c * IB * G + mm] = buf[l * RP + j + G * mm];
+      for (int mm = 0; mm < QL; ++mm) x[c * QL + mm] = buf[l * RP + j + G * mm];
       lds_barrier();
     }
   } else {
@@ -497,20 +508,23 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
   for (int c = 0; c < NCH; ++c) {
 #pragma unroll
     for (int kk = 0; kk < IB * G; ++kk)
-      if (c * IB * G + kk < M) buf[l * RP + kk * XS + j] = x[c * IB * G + kk];
+      if (c * IB * G + kk < M) buf[l * RP + kk * G + j] = x[c * IB * G + kk];
     lds_barrier();
 #pragma unroll
-    for (int ii = 0; ii < IB; ++ii)
+    for (int ii = 0; ii < IB; ++ii) {
+      // k1 = j + G*(c*IB + ii); unused slots (k1 >= M) read k1 = 0's values
+      int k1l = j + G * ii;
+      if (!EXACT && (c * IB + ii + 1) * G > M) k1l = (k1l + c * IB * G < M) ? k1l : 0;
 #pragma unroll
-      for (int jj = 0; jj < G; ++jj)
-        x[(c * IB + ii) * G + jj] = buf[l * RP + (j + G * ii) * XS + jj];
+      for (int jj = 0; jj < G; ++jj) x[(c * IB + ii) * G + jj] = buf[l * RP + k1l * G + jj];
+    }
     lds_barrier();
   }
 
   // ---------------- twiddle, step 2: x[i*G + k2] = X[j + G*i + M*k2] ----------------
   CT u[G];
 #pragma unroll
-  for (int jj = 1; jj < G; ++jj) u[jj] = tw[jj * j];   // W_NF^(jj*j)
+  for (int jj = 1; jj < G; ++jj) u[jj] = tw[(jj * j) % NF];   // W_NF^(jj*j)
 #pragma unroll
   for (int i = 0; i < NI; ++i) {
 #pragma unroll
@@ -519,7 +533,10 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
       if (i != 0) t = cmul(t, tw[G * ((jj * i) % M)]);   // W_M^(jj*i)
       x[i * G + jj] = t;
     }
-    dft<T, G>(x + i * G, sg);
+    if constexpr (G <= 8)
+      dft<T, G>(x + i * G, sg);
+    else
+      fft<T, G, 0, 1, M>(x + i * G, tw, sg);   // W_G^e = W_NF^(M*e)
   }
 
   // ---------------- store (trunc + output modulation) ----------------
@@ -541,7 +558,7 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
         const int row = e / WC, col = e - row * WC;
         const int64_t rb = dbase[row0 + row];
         const int kk = c * WC + col - zout;
-        if (rb >= 0 && c * WC + col < NF && kk >= 0 && kk < nout) {
+        if (rb >= 0 && kk >= 0 && kk < nout) {
           const CT v = buf[row * RP + col];
           const T f = modsign<T>(P.omod, kk);
           d[rb + kk] = CT{v.x * f, v.y * f};
@@ -581,6 +598,20 @@ __host__ __device__ constexpr bool small_length(int nf, bool f64) {
     case 2: case 3: case 4: case 5: case 6: case 8: case 10: case 12: case 15: case 16:
     case 20: case 24: case 30: case 32: case 40: case 48: case 60: case 64:
     case 80: case 96: case 120: case 128:
+      return true;
+    default:
+      return false;
+  }
+}
+
+// Lengths above 128 with a split-form instantiation for the per-pass launch:
+// 2^a 3^b 5^c, not powers of two, whose split holds at most 32 complex64
+// (16 complex128) values per lane.
+__host__ __device__ constexpr bool mid_length(int nf, bool f64) {
+  switch (nf) {
+    case 144: case 160: case 192: case 240:
+    case 200: case 216: case 288: case 320: case 384: case 400: case 432: case 480:
+    case 576: case 640: case 768: case 800: case 864: case 960:
       return true;
     default:
       return false;
