@@ -39,10 +39,11 @@ def parse():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--workload", default="cfg2",
-                    choices=["cfg2", "cfg3", "cfg4", "cfg5", "gcell"],
+                    choices=["cfg2", "cfg3", "cfg4", "cfg5", "gcell", "mixed"],
                     help="cfg2 (default): 512^3 FP64 r2c+c2r; cfg3: precision study "
                          "1024x1024x512 c2c; cfg4: 384 uneven c2c batch; cfg5: oversampled "
-                         "shifted r2c forward+adjoint")
+                         "shifted r2c forward+adjoint; gcell: ghost cells; mixed: "
+                         "mixed-radix c2c sides 240..960")
     return ap.parse_args()
 
 

@@ -11,6 +11,9 @@ cfg5  oversampled shifted r2c ni=128^3 -> n=256^3, loss sum w_k |X_k|^2:
       forward + adjoint (gradient) time, FP64 and FP32.
 gcell ghost-cell exchange and reduce of a 512^3 block with 4 ghost cells on
       every side (FP64 real, FP64 complex r2c array, FP32 real).
+mixed 3-d c2c forward transforms of mixed-radix sides (480^3, 384^3, 240^3
+      and 960x960x64) in FP64 and complex64: per-pass times and kernels, and
+      the relative L2 difference from torch.fft.fftn of the same input.
 Each prints one JSON line.  Inputs are generated on the GPU with torch
 (test-data plumbing) and resident in HBM before timing.
 """
@@ -241,5 +244,33 @@ def gcell(args):
     print(json.dumps(out))
 
 
+def mixed(args):
+    import torch
+    dev = torch.device("cuda:0")
+    rc, comm = P.create_procmesh([1, 1])
+    out = {"workload": "mixed-radix 3-d c2c forward", "cases": []}
+    for n in ([480, 480, 480], [384, 384, 384], [240, 240, 240], [960, 960, 64]):
+        for prec in ("f64", "f32"):
+            cdt = torch.complex128 if prec == "f64" else torch.complex64
+            g = torch.Generator(device=dev).manual_seed(args.seed)
+            x = torch.rand(n, device=dev, dtype=cdt, generator=g) * 2 - (1 + 1j)
+            y = torch.empty_like(x)
+            p = P.Plan("c2c", n, x.data_ptr(), y.data_ptr(), comm=comm, prec=prec)
+            assert p.ok
+            ms = _time(p.execute, 5, warm=2)
+            passes = _pass_times(p, x.data_ptr(), y.data_ptr())
+            ref = torch.fft.fftn(x)
+            err = (torch.linalg.vector_norm(y - ref) / torch.linalg.vector_norm(ref)).item()
+            pts = int(np.prod(n))
+            out["cases"].append({
+                "n": n, "prec": prec, "ms": round(ms, 3),
+                "GFLOPs": round(5 * pts * math.log2(pts) / (ms * 1e-3) / 1e9, 1),
+                "GBps_alg": round(3 * 2 * pts * x.element_size() / (ms * 1e-3) / 1e9, 1),
+                "rel_l2_vs_torch_fft": float("%.3g" % err), "passes": passes})
+            p.destroy()
+            del x, y, ref
+    print(json.dumps(out))
+
+
 def run(args):
-    {"cfg3": cfg3, "cfg4": cfg4, "cfg5": cfg5, "gcell": gcell}[args.workload](args)
+    {"cfg3": cfg3, "cfg4": cfg4, "cfg5": cfg5, "gcell": gcell, "mixed": mixed}[args.workload](args)
