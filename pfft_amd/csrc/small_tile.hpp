@@ -393,12 +393,14 @@ __device__ __forceinline__ void small_line_tile(const pfk_pass &P, const int64_t
 //   3. takes the G-point DFTs over jj: X[k1 + M*k2] = sum_jj W_G^(jj*k2) (..).
 // This is the two-factor split NF = G*M of the DFT (n = j + G*m,
 // k = k1 + M*k2) with the first factor over lanes.  Slots with k1 >= M
-// (when G does not divide M) are computed and dropped.  The lanes of a wave
-// take 64/G lines, line index fastest: lane = l + (64/G)*j, so that with
-// lines adjacent along the tile dimension each load or store instruction
-// covers G runs of 64/G adjacent lines.  Contiguous lines are staged through
-// LDS in chunks of wc columns, as the one-line-per-lane form does; the LDS
-// chunk of a wave is also the exchange buffer of step 2, done in the same
+// (when G does not divide M) are computed and dropped.  Threads take the
+// tile's LPT lines line index fastest: thread t is lane j = t / LPT of line
+// t % LPT, so that with lines adjacent along the tile dimension each load or
+// store instruction of a wave covers runs of min(LPT, 64) adjacent lines
+// (the G lanes of a line sit in different waves; the exchange is
+// workgroup-wide).  Contiguous lines are staged through LDS in chunks of wc
+// columns, each wave staging 64/G rows, as the one-line-per-lane form does;
+// the LDS rows are also the exchange buffer of step 2, done in the same
 // number of chunks.
 template <typename T, int NF>
 __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_t tile,
@@ -416,11 +418,11 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
                 Q == QL, "shape");
   static_assert(EXACT || NCH == 1, "uneven split in one chunk");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int l = lane % LW, j = lane / LW;   // line within the wave, lane within the line
-  const int row0 = wave * LW;               // first line of the wave within the tile
+  const int r = tid % LPT, j = tid / LPT;   // line within the tile, lane within the line
+  const int row0 = wave * LW;               // first line the wave stages
   int64_t nlines = P.text;
   for (int i = 0; i < P.nb; ++i) nlines *= P.bext[i];
-  CT *buf = reinterpret_cast<CT *>(smem) + wave * LW * RP;
+  CT *buf = reinterpret_cast<CT *>(smem);   // LPT rows of RP slots
   int64_t *sbase = reinterpret_cast<int64_t *>(smem + NW * LW * RP * sizeof(CT));
   int64_t *dbase = sbase + LPT;
   if (tid < LPT) {
@@ -443,7 +445,7 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
     dbase[tid] = db;
   }
   lds_barrier();
-  const int64_t sb = sbase[row0 + l], db = dbase[row0 + l];
+  const int64_t sb = sbase[r], db = dbase[r];
   const bool active = sb >= 0;
   const CT *s = reinterpret_cast<const CT *>(src);
   CT *d = reinterpret_cast<CT *>(dst);
@@ -474,11 +476,11 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
         const int jj = c * WC + col - zin;
         const bool ok = rb >= 0 && jj >= 0 && jj < nin;
         const T f = modsign<T>(P.imod, jj);
-        buf[row * RP + col] = ok ? CT{v[q].x * f, v[q].y * f} : CT{(T)0, (T)0};
+        buf[(row0 + row) * RP + col] = ok ? CT{v[q].x * f, v[q].y * f} : CT{(T)0, (T)0};
       }
       lds_barrier();
 #pragma unroll
-      for (int mm = 0; mm < QL; ++mm) x[c * QL + mm] = buf[l * RP + j + G * mm];
+      for (int mm = 0; mm < QL; ++mm) x[c * QL + mm] = buf[r * RP + j + G * mm];
       lds_barrier();
     }
   } else {
@@ -508,7 +510,7 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
   for (int c = 0; c < NCH; ++c) {
 #pragma unroll
     for (int kk = 0; kk < IB * G; ++kk)
-      if (c * IB * G + kk < M) buf[l * RP + kk * G + j] = x[c * IB * G + kk];
+      if (c * IB * G + kk < M) buf[r * RP + kk * G + j] = x[c * IB * G + kk];
     lds_barrier();
 #pragma unroll
     for (int ii = 0; ii < IB; ++ii) {
@@ -516,7 +518,7 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
       int k1l = j + G * ii;
       if (!EXACT && (c * IB + ii + 1) * G > M) k1l = (k1l + c * IB * G < M) ? k1l : 0;
 #pragma unroll
-      for (int jj = 0; jj < G; ++jj) x[(c * IB + ii) * G + jj] = buf[l * RP + k1l * G + jj];
+      for (int jj = 0; jj < G; ++jj) x[(c * IB + ii) * G + jj] = buf[r * RP + k1l * G + jj];
     }
     lds_barrier();
   }
@@ -549,7 +551,7 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
         for (int k2 = 0; k2 < G; ++k2) {
           const int base = G * i + M * k2;   // column of lane 0; lane j adds j
           const bool mine = EXACT ? (base >= c * WC && base < (c + 1) * WC) : (j + G * i < M);
-          if (mine) buf[l * RP + base - c * WC + j] = x[i * G + k2];
+          if (mine) buf[r * RP + base - c * WC + j] = x[i * G + k2];
         }
       lds_barrier();
 #pragma unroll
@@ -559,7 +561,7 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
         const int64_t rb = dbase[row0 + row];
         const int kk = c * WC + col - zout;
         if (rb >= 0 && kk >= 0 && kk < nout) {
-          const CT v = buf[row * RP + col];
+          const CT v = buf[(row0 + row) * RP + col];
           const T f = modsign<T>(P.omod, kk);
           d[rb + kk] = CT{v.x * f, v.y * f};
         }
