@@ -127,8 +127,8 @@ static int batch_launch(pfk_batch *b) {
 }
 
 static int task_class(const pfk_task *t) {
-  if (!g_generic_only && t->pass.variant != 1 && small_covers(&t->pass) &&
-      small_length((int)t->pass.nf, t->pass.prec == PFK_F64))
+  if (!g_generic_only && t->pass.variant != 1 && t->pass.kind == PFK_C2C &&
+      small_covers(&t->pass) && small_length((int)t->pass.nf, t->pass.prec == PFK_F64))
     return (int)t->pass.nf;
   return 0;
 }

@@ -14,11 +14,27 @@ k_small(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
   small_tile<T, NF>(P, tile_base + blockIdx.x, src, dst, smem_raw);
 }
 
+template <typename T, int NF, int KIND>
+__global__ void __launch_bounds__(256)
+k_small_real(const pfk_pass P, const int64_t tile_base, const T *__restrict__ src,
+             T *__restrict__ dst) {
+  extern __shared__ unsigned char smem_raw[];
+  small_split_real_tile<T, NF, KIND>(P, tile_base + blockIdx.x, src, dst, smem_raw);
+}
+
 int small_covers(const pfk_pass *p) {
   if (p->n1 || p->dec_in || p->dec_out || p->twn) return 0;   // steps of long lines
-  if (p->kind != PFK_C2C || p->hw || p->nf != p->n || p->nf > 4096) return 0;
+  if (p->nf > 4096) return 0;
   const int nf = (int)p->nf;
   const bool f64 = p->prec == PFK_F64;
+  if (p->kind == PFK_C2C) {
+    if (p->hw || p->nf != p->n) return 0;
+  } else if (p->kind == PFK_R2C || p->kind == PFK_C2R) {
+    // even real lines through the half-length complex line (one-chunk shapes)
+    if ((p->n & 1) || 2 * p->nf != p->n || !split_real_length(nf, f64)) return 0;
+  } else {
+    return 0;
+  }
   if (small_length(nf, f64)) return 1;
   return (mid_length(nf, f64) && small::lds_bytes(nf, f64) <= (size_t)g_max_lds) ? 1 : 0;
 }
@@ -36,8 +52,21 @@ static int launch_small_nf(const pfk_pass *p, const void *src, void *dst) {
     const int64_t chunk = (int64_t)1 << 30;
     for (int64_t base = 0; base < ntiles; base += chunk) {
       const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
-      hipLaunchKernelGGL((k_small<T, NF>), dim3((unsigned)nblk), dim3(small_threads(NF, F64)),
-                         small::lds_bytes(NF, F64), g_stream, *p, base, (const T *)src, (T *)dst);
+      const dim3 grid((unsigned)nblk), blk(small_threads(NF, F64));
+      const size_t lds = small::lds_bytes(NF, F64);
+      if (p->kind == PFK_C2C) {
+        hipLaunchKernelGGL((k_small<T, NF>), grid, blk, lds, g_stream, *p, base, (const T *)src,
+                           (T *)dst);
+      } else if constexpr (split_real_length(NF, F64)) {
+        if (p->kind == PFK_R2C)
+          hipLaunchKernelGGL((k_small_real<T, NF, PFK_R2C>), grid, blk, lds, g_stream, *p, base,
+                             (const T *)src, (T *)dst);
+        else
+          hipLaunchKernelGGL((k_small_real<T, NF, PFK_C2R>), grid, blk, lds, g_stream, *p, base,
+                             (const T *)src, (T *)dst);
+      } else {
+        return 1;
+      }
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) { set_error("pfk_launch_pass(small)", e); return -1; }
     }

@@ -383,6 +383,59 @@ __device__ __forceinline__ void small_line_tile(const pfk_pass &P, const int64_t
   }
 }
 
+// Steps 1-3 of the split form on the lane's registers: x[m] = line[j + G*m]
+// (m < M) in, x[i*G + k2] = X[j + G*i + M*k2] out.  buf holds the tile's
+// rows (row r of RP slots); the exchange ends with a barrier.
+template <typename T, int NF>
+__device__ __forceinline__ void split_fft(small::C2<T> *x, small::C2<T> *buf, const int r,
+                                          const int j, const T sg,
+                                          const small::C2<T> *__restrict__ tw) {
+  using namespace small;
+  using CT = C2<T>;
+  constexpr split_shape SH = split_of(NF, sizeof(T) == 8);
+  constexpr int G = SH.g, M = SH.m, NI = SH.ni, IB = SH.ib, NCH = SH.nch, RP = SH.rp;
+  constexpr bool EXACT = M % G == 0;
+  // ---------------- step 1: M-point DFT of the lane's sequence ----------------
+  fft<T, M, 0, 1, G>(x, tw, sg);
+
+  // ---------------- exchange: x[i*G + jj] = Y_jj[j + G*i] ----------------
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+    for (int kk = 0; kk < IB * G; ++kk)
+      if (c * IB * G + kk < M) buf[r * RP + kk * G + j] = x[c * IB * G + kk];
+    lds_barrier();
+#pragma unroll
+    for (int ii = 0; ii < IB; ++ii) {
+      // k1 = j + G*(c*IB + ii); unused slots (k1 >= M) read k1 = 0's values
+      int k1l = j + G * ii;
+      if (!EXACT && (c * IB + ii + 1) * G > M) k1l = (k1l + c * IB * G < M) ? k1l : 0;
+#pragma unroll
+      for (int jj = 0; jj < G; ++jj) x[(c * IB + ii) * G + jj] = buf[r * RP + k1l * G + jj];
+    }
+    lds_barrier();
+  }
+
+  // ---------------- twiddle, step 2: x[i*G + k2] = X[j + G*i + M*k2] ----------------
+  CT u[G];
+#pragma unroll
+  for (int jj = 1; jj < G; ++jj) u[jj] = tw[(jj * j) % NF];   // W_NF^(jj*j)
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+#pragma unroll
+    for (int jj = 1; jj < G; ++jj) {
+      CT t = cmul(x[i * G + jj], u[jj]);
+      if (i != 0) t = cmul(t, tw[G * ((jj * i) % M)]);   // W_M^(jj*i)
+      x[i * G + jj] = t;
+    }
+    if constexpr (G <= 8)
+      dft<T, G>(x + i * G, sg);
+    else
+      fft<T, G, 0, 1, M>(x + i * G, tw, sg);   // W_G^e = W_NF^(M*e)
+  }
+
+}
+
 // One tile (NW*64/G lines) of a c2c pass of length NF = G*M, G lanes per
 // line.  Lane j (0 <= j < G) of a line holds the decimated sequence
 // x[j + G*m], m in [0, M), and:
@@ -500,46 +553,7 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
     }
   }
 
-  // ---------------- step 1: M-point DFT of the lane's sequence ----------------
-  const T sg = (T)P.sign;
-  const CT *tw = reinterpret_cast<const CT *>(P.tw);
-  fft<T, M, 0, 1, G>(x, tw, sg);
-
-  // ---------------- exchange: x[i*G + jj] = Y_jj[j + G*i] ----------------
-#pragma unroll
-  for (int c = 0; c < NCH; ++c) {
-#pragma unroll
-    for (int kk = 0; kk < IB * G; ++kk)
-      if (c * IB * G + kk < M) buf[r * RP + kk * G + j] = x[c * IB * G + kk];
-    lds_barrier();
-#pragma unroll
-    for (int ii = 0; ii < IB; ++ii) {
-      // k1 = j + G*(c*IB + ii); unused slots (k1 >= M) read k1 = 0's values
-      int k1l = j + G * ii;
-      if (!EXACT && (c * IB + ii + 1) * G > M) k1l = (k1l + c * IB * G < M) ? k1l : 0;
-#pragma unroll
-      for (int jj = 0; jj < G; ++jj) x[(c * IB + ii) * G + jj] = buf[r * RP + k1l * G + jj];
-    }
-    lds_barrier();
-  }
-
-  // ---------------- twiddle, step 2: x[i*G + k2] = X[j + G*i + M*k2] ----------------
-  CT u[G];
-#pragma unroll
-  for (int jj = 1; jj < G; ++jj) u[jj] = tw[(jj * j) % NF];   // W_NF^(jj*j)
-#pragma unroll
-  for (int i = 0; i < NI; ++i) {
-#pragma unroll
-    for (int jj = 1; jj < G; ++jj) {
-      CT t = cmul(x[i * G + jj], u[jj]);
-      if (i != 0) t = cmul(t, tw[G * ((jj * i) % M)]);   // W_M^(jj*i)
-      x[i * G + jj] = t;
-    }
-    if constexpr (G <= 8)
-      dft<T, G>(x + i * G, sg);
-    else
-      fft<T, G, 0, 1, M>(x + i * G, tw, sg);   // W_G^e = W_NF^(M*e)
-  }
+  split_fft<T, NF>(x, buf, r, j, (T)P.sign, reinterpret_cast<const CT *>(P.tw));
 
   // ---------------- store (trunc + output modulation) ----------------
   if (lds_out) {
@@ -580,6 +594,180 @@ __device__ __forceinline__ void small_split_tile(const pfk_pass &P, const int64_
         }
       }
   }
+}
+
+// Line bases of the tile's LPT lines into sbase / dbase (-1 past the end),
+// by the first LPT threads; the caller synchronises.
+template <int LPT>
+__device__ __forceinline__ void tile_bases(const pfk_pass &P, const int64_t tile,
+                                           int64_t *sbase, int64_t *dbase) {
+  const int tid = threadIdx.x;
+  int64_t nlines = P.text;
+  for (int i = 0; i < P.nb; ++i) nlines *= P.bext[i];
+  if (tid < LPT) {
+    const int64_t line = tile * LPT + tid;
+    int64_t sb = -1, db = -1;
+    if (line < nlines) {
+      const int64_t c = line % P.text;
+      int64_t bflat = line / P.text;
+      sb = c * P.tsin;
+      db = c * P.tsout;
+      for (int i = 0; i < P.nb; ++i) {
+        const int64_t e = P.bext[i];
+        const int64_t d = bflat % e;
+        bflat /= e;
+        sb += d * P.bsin[i];
+        db += d * P.bsout[i];
+      }
+    }
+    sbase[tid] = sb;
+    dbase[tid] = db;
+  }
+}
+
+// Visits the (row, col) slots of the tile's LPT rows x ncols columns: with
+// the line contiguous (unit element stride) each wave walks its own rows
+// column-fastest; otherwise the workgroup walks the slots row-fastest
+// (adjacent lines together).
+template <int LPT, int LW, int NT, typename F>
+__device__ __forceinline__ void tile_slots(const bool col_fast, const int ncols, F &&f) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (col_fast) {
+    for (int rr = 0; rr < LW; ++rr)
+      for (int col = lane; col < ncols; col += 64) f(wave * LW + rr, col);
+  } else {
+    const int tot = LPT * ncols;
+    for (int idx = tid; idx < tot; idx += NT) f(idx % LPT, idx / LPT);
+  }
+}
+
+// One tile of an even-length r2c or c2r pass (n = 2*NF, P.nf = NF) in the
+// split form; the shape must have one chunk (wc = NF, rows of NF+1 slots).
+// r2c: the real line is staged into the LDS rows (real slot p = half of
+// complex slot p/2), the lanes take z_m = x_2m + i x_2m+1 at m = j + G*m',
+// the split FFT gives Z, which goes back to the rows, and the output slots
+// combine Z_k and Z_{NF-k}:
+//   X_k = (Z_k + conj Z_{NF-k})/2 + W_n^k (Z_k - conj Z_{NF-k})/(2i).
+// c2r: the half spectrum is staged, the lanes form
+//   Z_k = E_k + i O_k, E_k = X_k + conj X_{NF-k}, O_k = (X_k - conj X_{NF-k}) W_n^k
+// (Z_0 = (X_0 + X_NF, X_0 - X_NF) from the real parts), the split FFT gives
+// z, which goes back to the rows, and real output j is slot j of the rows'
+// real view.  Embed, trunc, modulation and the Hermitian weights of the
+// adjoint passes follow the general kernel.
+template <typename T, int NF, int KIND>
+__device__ __forceinline__ void small_split_real_tile(const pfk_pass &P, const int64_t tile,
+                                                      const T *__restrict__ src,
+                                                      T *__restrict__ dst, unsigned char *smem) {
+  using namespace small;
+  using CT = C2<T>;
+  constexpr split_shape SH = split_of(NF, sizeof(T) == 8);
+  constexpr int G = SH.g, M = SH.m, NI = SH.ni, LW = SH.lw, LPT = SH.lpt, NW = SH.nw,
+                RP = SH.rp;
+  constexpr int NT = NW * 64;
+  static_assert(G > 1 && SH.nch == 1 && SH.wc == NF, "one-chunk split shape");
+  const int tid = threadIdx.x;
+  const int r = tid % LPT, j = tid / LPT;
+  CT *buf = reinterpret_cast<CT *>(smem);
+  T *rbuf = reinterpret_cast<T *>(smem);   // real view: row r starts at 2*r*RP
+  int64_t *sbase = reinterpret_cast<int64_t *>(smem + NW * LW * RP * sizeof(CT));
+  int64_t *dbase = sbase + LPT;
+  tile_bases<LPT>(P, tile, sbase, dbase);
+  lds_barrier();
+  const int nin = (int)P.nin, zin = (int)P.zin, nout = (int)P.nout, zout = (int)P.zout;
+  const CT *rtw = reinterpret_cast<const CT *>(P.rtw);
+
+  // ---------------- stage in ----------------
+  if constexpr (KIND == PFK_R2C) {
+    if (zin > 0 || nin < 2 * NF)
+      for (int idx = tid; idx < LPT * 2 * NF; idx += NT) {
+        const int row = idx / (2 * NF), p = idx - row * (2 * NF);
+        if (p < zin || p >= zin + nin) rbuf[2 * row * RP + p] = (T)0;
+      }
+    tile_slots<LPT, LW, NT>(P.sin == 1, nin, [&](int row, int jj) {
+      const int64_t b = sbase[row];
+      const T v = b >= 0 ? src[b + (int64_t)jj * P.sin] : (T)0;
+      rbuf[2 * row * RP + zin + jj] = v * modsign<T>(P.imod, jj);
+    });
+  } else {
+    const CT *s = reinterpret_cast<const CT *>(src);
+    if (zin > 0 || nin < NF + 1)
+      for (int idx = tid; idx < LPT * (NF + 1); idx += NT) {
+        const int row = idx / (NF + 1), p = idx - row * (NF + 1);
+        if (p < zin || p >= zin + nin) buf[row * RP + p] = CT{(T)0, (T)0};
+      }
+    tile_slots<LPT, LW, NT>(P.sin == 1, nin, [&](int row, int jj) {
+      const int64_t b = sbase[row];
+      const CT v = b >= 0 ? s[b + (int64_t)jj * P.sin] : CT{(T)0, (T)0};
+      T f = modsign<T>(P.imod, jj);
+      const int p = zin + jj;
+      if (P.hw == 1 && p != 0 && p != NF) f *= (T)0.5;
+      buf[row * RP + p] = CT{v.x * f, v.y * f};
+    });
+  }
+  lds_barrier();
+
+  // ---------------- lane values ----------------
+  CT x[NI * G];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int q = j + G * m;
+    if constexpr (KIND == PFK_R2C) {
+      x[m] = buf[r * RP + q];
+    } else {
+      const CT a = buf[r * RP + q];
+      const CT b = buf[r * RP + NF - q];
+      if (q == 0) {
+        x[m] = CT{a.x + b.x, a.x - b.x};   // b = X_NF here
+      } else {
+        const CT E{a.x + b.x, a.y - b.y};
+        const CT O = cmul(CT{a.x - b.x, a.y + b.y}, rtw[q]);
+        x[m] = CT{E.x - O.y, E.y + O.x};
+      }
+    }
+  }
+  lds_barrier();   // the exchange reuses the rows
+
+  split_fft<T, NF>(x, buf, r, j, (T)P.sign, reinterpret_cast<const CT *>(P.tw));
+
+#pragma unroll
+  for (int i = 0; i < NI; ++i)
+#pragma unroll
+    for (int k2 = 0; k2 < G; ++k2) {
+      const int k1 = j + G * i;
+      if (M % G == 0 || k1 < M) buf[r * RP + k1 + M * k2] = x[i * G + k2];
+    }
+  lds_barrier();
+
+  // ---------------- stage out ----------------
+  if constexpr (KIND == PFK_R2C) {
+    CT *d = reinterpret_cast<CT *>(dst);
+    tile_slots<LPT, LW, NT>(P.sout == 1, nout, [&](int row, int k) {
+      const int64_t b = dbase[row];
+      if (b < 0) return;
+      const int kk = zout + k;
+      const CT a = buf[row * RP + (kk == NF ? 0 : kk)];
+      const CT c = buf[row * RP + (kk == 0 ? 0 : NF - kk)];
+      const CT E{(a.x + c.x) * (T)0.5, (a.y - c.y) * (T)0.5};
+      const CT O{(a.y + c.y) * (T)0.5, -(a.x - c.x) * (T)0.5};
+      const CT WO = cmul(O, rtw[kk]);
+      T f = modsign<T>(P.omod, k);
+      if (P.hw == 2 && kk != 0 && kk != NF) f *= (T)2;
+      d[b + (int64_t)k * P.sout] = CT{(E.x + WO.x) * f, (E.y + WO.y) * f};
+    });
+  } else {
+    tile_slots<LPT, LW, NT>(P.sout == 1, nout, [&](int row, int k) {
+      const int64_t b = dbase[row];
+      if (b < 0) return;
+      dst[b + (int64_t)k * P.sout] = rbuf[2 * row * RP + zout + k] * modsign<T>(P.omod, k);
+    });
+  }
+}
+
+// Whether the split form computes even-length real passes of length NF
+// (nf = NF, n = 2*NF): a split shape of one chunk.
+__host__ __device__ constexpr bool split_real_length(int nf, bool f64) {
+  const small::split_shape s = small::split_of(nf, f64);
+  return s.g > 1 && s.nch == 1 && s.wc == nf;
 }
 
 // One tile of a c2c pass of length NF: small_lines_per_tile(NF) lines.
