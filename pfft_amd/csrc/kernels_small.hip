@@ -1,7 +1,9 @@
 // kernels_small.hip -- per-pass launch of the register-resident short-length
-// c2c kernel (small_tile.hpp): one line per lane (256 lines per workgroup)
-// or G lanes per line (256/G lines) for the longer lengths, including the
-// mid lengths 144..960 that only this per-pass launch instantiates.
+// kernel (small_tile.hpp): c2c passes one line per lane (256 lines per
+// workgroup) or G lanes per line (256/G lines) for the longer lengths,
+// including the mid lengths 144..960 that only this per-pass launch
+// instantiates; and even-length r2c / c2r passes whose half length has a
+// one-chunk split (small_split_real_tile).
 #include "small_tile.hpp"
 
 namespace pfk {
