@@ -677,32 +677,68 @@ __device__ __forceinline__ void small_split_real_tile(const pfk_pass &P, const i
   const CT *rtw = reinterpret_cast<const CT *>(P.rtw);
 
   // ---------------- stage in ----------------
+  // Every slot of the tile's rows is written (zero outside the embedded
+  // input); all loads of a thread are issued before its LDS writes.  With
+  // the line contiguous each wave fills its own rows column-fastest, else
+  // the workgroup fills the rows row-fastest.
+  const bool col_fast = P.sin == 1;
+  const int lane = tid & 63, row0 = (tid >> 6) * LW;
   if constexpr (KIND == PFK_R2C) {
-    if (zin > 0 || nin < 2 * NF)
-      for (int idx = tid; idx < LPT * 2 * NF; idx += NT) {
-        const int row = idx / (2 * NF), p = idx - row * (2 * NF);
-        if (p < zin || p >= zin + nin) rbuf[2 * row * RP + p] = (T)0;
-      }
-    tile_slots<LPT, LW, NT>(P.sin == 1, nin, [&](int row, int jj) {
+    constexpr int NC = 2 * NF;   // real slots per row
+    constexpr int Q = 2 * M;     // = LW*NC/64 = LPT*NC/NT
+    static_assert(Q * 64 == LW * NC && Q * NT == LPT * NC, "stage-in split");
+    T v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = col_fast ? lane + 64 * q : tid + NT * q;
+      const int row = col_fast ? row0 + e / NC : e % LPT;
+      const int col = col_fast ? e % NC : e / LPT;
       const int64_t b = sbase[row];
-      const T v = b >= 0 ? src[b + (int64_t)jj * P.sin] : (T)0;
-      rbuf[2 * row * RP + zin + jj] = v * modsign<T>(P.imod, jj);
-    });
+      const int jj = col - zin;
+      const bool ok = b >= 0 && jj >= 0 && jj < nin;
+      v[q] = src[ok ? b + (int64_t)jj * P.sin : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = col_fast ? lane + 64 * q : tid + NT * q;
+      const int row = col_fast ? row0 + e / NC : e % LPT;
+      const int col = col_fast ? e % NC : e / LPT;
+      const int jj = col - zin;
+      const bool ok = sbase[row] >= 0 && jj >= 0 && jj < nin;
+      rbuf[2 * row * RP + col] = ok ? v[q] * modsign<T>(P.imod, jj) : (T)0;
+    }
   } else {
     const CT *s = reinterpret_cast<const CT *>(src);
-    if (zin > 0 || nin < NF + 1)
-      for (int idx = tid; idx < LPT * (NF + 1); idx += NT) {
-        const int row = idx / (NF + 1), p = idx - row * (NF + 1);
-        if (p < zin || p >= zin + nin) buf[row * RP + p] = CT{(T)0, (T)0};
-      }
-    tile_slots<LPT, LW, NT>(P.sin == 1, nin, [&](int row, int jj) {
+    constexpr int NC = NF + 1;                       // complex slots per row
+    constexpr int QW = (LW * NC + 63) / 64;          // per lane, wave-own rows
+    constexpr int QT = (LPT * NC + NT - 1) / NT;     // per thread, row-fastest
+    constexpr int Q = QW > QT ? QW : QT;
+    CT v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = col_fast ? lane + 64 * q : tid + NT * q;
+      const bool in = col_fast ? (q < QW && e < LW * NC) : (q < QT && e < LPT * NC);
+      const int row = in ? (col_fast ? row0 + e / NC : e % LPT) : 0;
+      const int col = col_fast ? e % NC : e / LPT;
       const int64_t b = sbase[row];
-      const CT v = b >= 0 ? s[b + (int64_t)jj * P.sin] : CT{(T)0, (T)0};
-      T f = modsign<T>(P.imod, jj);
-      const int p = zin + jj;
-      if (P.hw == 1 && p != 0 && p != NF) f *= (T)0.5;
-      buf[row * RP + p] = CT{v.x * f, v.y * f};
-    });
+      const int jj = col - zin;
+      const bool ok = in && b >= 0 && jj >= 0 && jj < nin;
+      v[q] = s[ok ? b + (int64_t)jj * P.sin : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = col_fast ? lane + 64 * q : tid + NT * q;
+      const bool in = col_fast ? (q < QW && e < LW * NC) : (q < QT && e < LPT * NC);
+      if (in) {
+        const int row = col_fast ? row0 + e / NC : e % LPT;
+        const int col = col_fast ? e % NC : e / LPT;
+        const int jj = col - zin;
+        const bool ok = sbase[row] >= 0 && jj >= 0 && jj < nin;
+        T f = modsign<T>(P.imod, jj);
+        if (P.hw == 1 && col != 0 && col != NF) f *= (T)0.5;
+        buf[row * RP + col] = ok ? CT{v[q].x * f, v[q].y * f} : CT{(T)0, (T)0};
+      }
+    }
   }
   lds_barrier();
 
