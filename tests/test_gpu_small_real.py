@@ -99,3 +99,22 @@ def test_real_split_adjoint_dot_product(nf, prec):
         lhs, rhs = float(fx @ y), float(x @ fty)
         t = 1e-10 if prec == "f64" else 1e-4
         assert abs(lhs - rhs) <= t * (abs(lhs) + abs(rhs)), (kind, lhs, rhs)
+
+
+@pytest.mark.parametrize("kind,flags", [("r2c", P.PADDED_R2C), ("r2c", TO | P.PADDED_R2C),
+                                        ("c2r", P.PADDED_C2R), ("c2r", TI | P.PADDED_C2R)])
+@pytest.mark.parametrize("nf", [60, 240])
+def test_real_split_padded_inplace(kind, flags, nf):
+    spec = spec_of(kind, [6, 4, 2 * nf], sign=(+1 if kind == "c2r" else -1), flags=flags)
+    x = random_input(spec, 11)
+    got = run_gpu(spec, x, inplace=True, plan_flags_extra=EST)
+    assert rel_l2(got, oracle_out(spec, x), out_mask(spec)) <= 1e-12
+
+
+@pytest.mark.parametrize("kind,flags", [("r2c", TO), ("c2r", TI), ("c2c", 0), ("c2c", TO)])
+def test_split_howmany(kind, flags):
+    spec = spec_of(kind, [5, 4, 480], howmany=3, sign=(+1 if kind == "c2r" else -1),
+                   flags=flags)
+    x = random_input(spec, 12)
+    got = run_gpu(spec, x, plan_flags_extra=EST)
+    assert rel_l2(got, oracle_out(spec, x), out_mask(spec)) <= 1e-12
