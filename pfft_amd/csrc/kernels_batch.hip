@@ -66,19 +66,31 @@ k_batch_small(const pfk_task *__restrict__ tasks, const int t0, const int t1,
   }
 }
 
+// short lengths the batch runs with the short-length kernel: those whose
+// workgroups have 4 waves (the batch launches 256 threads)
+__host__ __device__ constexpr bool batch_small_length(int nf, bool f64) {
+  return small_length(nf, f64) && small_threads(nf, f64) == 256;
+}
+
 // kernel of a task group: 0 = general, else the short length NF
 template <typename T>
 static const void *batch_kernel(int cls) {
   if (cls == 0) return reinterpret_cast<const void *>(&k_batch_generic<T>);
   switch (cls) {
 #define PFK_BATCH_SMALL(N) \
-  case N: if constexpr (small_length(N, sizeof(T) == 8)) return reinterpret_cast<const void *>(&k_batch_small<T, N>); break;
+  case N:                                                                               \
+    if constexpr (batch_small_length(N, sizeof(T) == 8))                                \
+      return reinterpret_cast<const void *>(&k_batch_small<T, N>);                      \
+    break;
     PFK_BATCH_SMALL(2) PFK_BATCH_SMALL(3) PFK_BATCH_SMALL(4) PFK_BATCH_SMALL(5)
     PFK_BATCH_SMALL(6) PFK_BATCH_SMALL(8) PFK_BATCH_SMALL(10) PFK_BATCH_SMALL(12)
     PFK_BATCH_SMALL(15) PFK_BATCH_SMALL(16) PFK_BATCH_SMALL(20) PFK_BATCH_SMALL(24)
     PFK_BATCH_SMALL(30) PFK_BATCH_SMALL(32) PFK_BATCH_SMALL(40) PFK_BATCH_SMALL(48)
     PFK_BATCH_SMALL(60) PFK_BATCH_SMALL(64) PFK_BATCH_SMALL(80) PFK_BATCH_SMALL(96)
     PFK_BATCH_SMALL(120) PFK_BATCH_SMALL(128)
+    PFK_BATCH_SMALL(9) PFK_BATCH_SMALL(18) PFK_BATCH_SMALL(36) PFK_BATCH_SMALL(50)
+    PFK_BATCH_SMALL(54) PFK_BATCH_SMALL(72) PFK_BATCH_SMALL(90) PFK_BATCH_SMALL(100)
+    PFK_BATCH_SMALL(108)
 #undef PFK_BATCH_SMALL
     default: break;
   }
@@ -128,7 +140,7 @@ static int batch_launch(pfk_batch *b) {
 
 static int task_class(const pfk_task *t) {
   if (!g_generic_only && t->pass.variant != 1 && t->pass.kind == PFK_C2C &&
-      small_covers(&t->pass) && small_length((int)t->pass.nf, t->pass.prec == PFK_F64))
+      small_covers(&t->pass) && batch_small_length((int)t->pass.nf, t->pass.prec == PFK_F64))
     return (int)t->pass.nf;
   return 0;
 }

@@ -817,14 +817,18 @@ __device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile
     small_line_tile<T, NF>(P, tile, src, dst, smem);
 }
 
-// Lengths with a small_tile instantiation (the same in both precisions).
+// Lengths with a small_tile instantiation: 2^a 3^b 5^c up to 128 except
+// the odd ones above 16 (one line per lane would not fit the registers),
+// and 90 (45 values per lane) in complex64 only.
 __host__ __device__ constexpr bool small_length(int nf, bool f64) {
-  (void)f64;
   switch (nf) {
-    case 2: case 3: case 4: case 5: case 6: case 8: case 10: case 12: case 15: case 16:
-    case 20: case 24: case 30: case 32: case 40: case 48: case 60: case 64:
-    case 80: case 96: case 120: case 128:
+    case 2: case 3: case 4: case 5: case 6: case 8: case 9: case 10: case 12: case 15:
+    case 16: case 18: case 20: case 24: case 30: case 32: case 36: case 40: case 48:
+    case 50: case 54: case 60: case 64: case 72: case 80: case 96: case 100: case 108:
+    case 120: case 128:
       return true;
+    case 90:
+      return !f64;
     default:
       return false;
   }
@@ -858,6 +862,9 @@ __device__ __forceinline__ void small_dispatch(const pfk_pass &P, const int64_t 
     PFK_SMALL_CASE(32) PFK_SMALL_CASE(40) PFK_SMALL_CASE(48) PFK_SMALL_CASE(60)
     PFK_SMALL_CASE(64) PFK_SMALL_CASE(80) PFK_SMALL_CASE(96) PFK_SMALL_CASE(120)
     PFK_SMALL_CASE(128)
+    PFK_SMALL_CASE(9) PFK_SMALL_CASE(18) PFK_SMALL_CASE(36) PFK_SMALL_CASE(50)
+    PFK_SMALL_CASE(54) PFK_SMALL_CASE(72) PFK_SMALL_CASE(90) PFK_SMALL_CASE(100)
+    PFK_SMALL_CASE(108)
 #undef PFK_SMALL_CASE
     default: break;
   }

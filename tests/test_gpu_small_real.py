@@ -16,8 +16,8 @@ pytestmark = pytest.mark.gpu
 
 TO, TI, SI, SO, EST = (P.TRANSPOSED_OUT, P.TRANSPOSED_IN, P.SHIFTED_IN, P.SHIFTED_OUT,
                        P.ESTIMATE)
-F32 = [20, 24, 30, 32, 48, 60, 96, 120, 144, 240, 480, 960]
-F64 = [30, 60, 96, 144, 240, 480, 960]
+F32 = [18, 20, 24, 30, 32, 36, 48, 50, 60, 72, 90, 96, 100, 120, 144, 240, 480, 960]
+F64 = [18, 30, 36, 54, 60, 72, 96, 108, 144, 240, 480, 960]
 
 
 def tol(prec):
