@@ -12,8 +12,9 @@ cfg5  oversampled shifted r2c ni=128^3 -> n=256^3, loss sum w_k |X_k|^2:
 gcell ghost-cell exchange and reduce of a 512^3 block with 4 ghost cells on
       every side (FP64 real, FP64 complex r2c array, FP32 real).
 mixed 3-d c2c forward transforms of mixed-radix sides (480^3, 384^3, 240^3
-      and 960x960x64) in FP64 and complex64: per-pass times and kernels, and
-      the relative L2 difference from torch.fft.fftn of the same input.
+      and 960x960x64) and r2c forward transforms (480^3, 384^3) in FP64 and
+      complex64: per-pass times and kernels, and the relative L2 difference
+      from torch.fft.fftn / rfftn of the same input.
 Each prints one JSON line.  Inputs are generated on the GPU with torch
 (test-data plumbing) and resident in HBM before timing.
 """
@@ -248,7 +249,7 @@ def mixed(args):
     import torch
     dev = torch.device("cuda:0")
     rc, comm = P.create_procmesh([1, 1])
-    out = {"workload": "mixed-radix 3-d c2c forward", "cases": []}
+    out = {"workload": "mixed-radix 3-d c2c and r2c forward", "cases": []}
     for n in ([480, 480, 480], [384, 384, 384], [240, 240, 240], [960, 960, 64]):
         for prec in ("f64", "f32"):
             cdt = torch.complex128 if prec == "f64" else torch.complex64
@@ -263,9 +264,29 @@ def mixed(args):
             err = (torch.linalg.vector_norm(y - ref) / torch.linalg.vector_norm(ref)).item()
             pts = int(np.prod(n))
             out["cases"].append({
-                "n": n, "prec": prec, "ms": round(ms, 3),
+                "kind": "c2c", "n": n, "prec": prec, "ms": round(ms, 3),
                 "GFLOPs": round(5 * pts * math.log2(pts) / (ms * 1e-3) / 1e9, 1),
                 "GBps_alg": round(3 * 2 * pts * x.element_size() / (ms * 1e-3) / 1e9, 1),
+                "rel_l2_vs_torch_fft": float("%.3g" % err), "passes": passes})
+            p.destroy()
+            del x, y, ref
+    for n in ([480, 480, 480], [384, 384, 384]):
+        for prec in ("f64", "f32"):
+            rdt = torch.float64 if prec == "f64" else torch.float32
+            cdt = torch.complex128 if prec == "f64" else torch.complex64
+            g = torch.Generator(device=dev).manual_seed(args.seed)
+            x = torch.rand(n, device=dev, dtype=rdt, generator=g) * 2 - 1
+            y = torch.empty(n[:-1] + [n[-1] // 2 + 1], device=dev, dtype=cdt)
+            p = P.Plan("r2c", n, x.data_ptr(), y.data_ptr(), comm=comm, prec=prec)
+            assert p.ok
+            ms = _time(p.execute, 5, warm=2)
+            passes = _pass_times(p, x.data_ptr(), y.data_ptr())
+            ref = torch.fft.rfftn(x)
+            err = (torch.linalg.vector_norm(y - ref) / torch.linalg.vector_norm(ref)).item()
+            pts = int(np.prod(n))
+            out["cases"].append({
+                "kind": "r2c", "n": n, "prec": prec, "ms": round(ms, 3),
+                "GFLOPs": round(2.5 * pts * math.log2(pts) / (ms * 1e-3) / 1e9, 1),
                 "rel_l2_vs_torch_fft": float("%.3g" % err), "passes": passes})
             p.destroy()
             del x, y, ref
