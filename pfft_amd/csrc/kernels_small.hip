@@ -37,8 +37,9 @@ int small_covers(const pfk_pass *p) {
   } else {
     return 0;
   }
-  if (small_length(nf, f64)) return 1;
-  return (mid_length(nf, f64) && small::lds_bytes(nf, f64) <= (size_t)g_max_lds) ? 1 : 0;
+  const bool real = p->kind != PFK_C2C;
+  if (small::lds_bytes(nf, f64, real) > (size_t)g_max_lds) return 0;
+  return (small_length(nf, f64) || mid_length(nf, f64)) ? 1 : 0;
 }
 
 template <typename T, int NF>
@@ -49,13 +50,14 @@ static int launch_small_nf(const pfk_pass *p, const void *src, void *dst) {
     int64_t nlines = p->text;
     for (int i = 0; i < p->nb; ++i) nlines *= p->bext[i];
     constexpr bool F64 = sizeof(T) == 8;
-    constexpr int lpt = small_lines_per_tile(NF, F64);
+    const bool real = p->kind != PFK_C2C;   // real passes take the one-chunk shape
+    const int lpt = small_lines_per_tile(NF, F64, real);
     const int64_t ntiles = (nlines + lpt - 1) / lpt;
     const int64_t chunk = (int64_t)1 << 30;
     for (int64_t base = 0; base < ntiles; base += chunk) {
       const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
-      const dim3 grid((unsigned)nblk), blk(small_threads(NF, F64));
-      const size_t lds = small::lds_bytes(NF, F64);
+      const dim3 grid((unsigned)nblk), blk(small_threads(NF, F64, real));
+      const size_t lds = small::lds_bytes(NF, F64, real);
       if (p->kind == PFK_C2C) {
         hipLaunchKernelGGL((k_small<T, NF>), grid, blk, lds, g_stream, *p, base, (const T *)src,
                            (T *)dst);

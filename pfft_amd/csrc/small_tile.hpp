@@ -189,7 +189,7 @@ struct split_shape {
 // two per-lane counts, M and ceil(M/G)*G.  When G divides M the line is
 // staged and exchanged in chunks of at most 8 KiB per wave (16 complex64
 // or 8 complex128 slots per lane); otherwise in one chunk of NF slots.
-__host__ __device__ constexpr split_shape split_of(int nf, bool f64) {
+__host__ __device__ constexpr split_shape split_of(int nf, bool f64, bool one = false) {
   int g = 1;
   if (nf > 128) {
     int best = 1 << 30;
@@ -210,7 +210,7 @@ __host__ __device__ constexpr split_shape split_of(int nf, bool f64) {
   s.m = nf / g;
   s.ni = (s.m + g - 1) / g;
   s.lw = 64 / g;
-  if (s.m % g == 0) {
+  if (s.m % g == 0 && !one) {
     const int wmax = (f64 ? 8 : 16) * g;
     s.ib = 1;
     for (int d = 1; d <= s.ni; ++d)
@@ -218,7 +218,7 @@ __host__ __device__ constexpr split_shape split_of(int nf, bool f64) {
     s.wc = s.ib * g * g;
     s.nch = s.ni / s.ib;
   } else {
-    s.ib = s.ni;
+    s.ib = s.ni;   // one chunk (for an exact split, ni*g*g = nf)
     s.wc = nf;
     s.nch = 1;
   }
@@ -231,8 +231,8 @@ __host__ __device__ constexpr split_shape split_of(int nf, bool f64) {
 }
 
 // LDS bytes one workgroup needs for length nf
-__host__ __device__ constexpr size_t lds_bytes(int nf, bool f64) {
-  const split_shape s = split_of(nf, f64);
+__host__ __device__ constexpr size_t lds_bytes(int nf, bool f64, bool one = false) {
+  const split_shape s = split_of(nf, f64, one);
   const size_t csz = f64 ? 16 : 8;
   if (s.g == 1) return 4 * 64 * (size_t)(128 / csz + 1) * csz + 2 * 256 * sizeof(int64_t);
   return (size_t)s.nw * s.lw * s.rp * csz + 2 * (size_t)s.lpt * sizeof(int64_t);
@@ -241,13 +241,13 @@ __host__ __device__ constexpr size_t lds_bytes(int nf, bool f64) {
 }  // namespace small
 
 // lines one workgroup tile of the short-length kernel covers
-__host__ __device__ constexpr int small_lines_per_tile(int nf, bool f64) {
-  return small::split_of(nf, f64).lpt;
+__host__ __device__ constexpr int small_lines_per_tile(int nf, bool f64, bool one = false) {
+  return small::split_of(nf, f64, one).lpt;
 }
 
 // threads per workgroup of the short-length kernel
-__host__ __device__ constexpr int small_threads(int nf, bool f64) {
-  return 64 * small::split_of(nf, f64).nw;
+__host__ __device__ constexpr int small_threads(int nf, bool f64, bool one = false) {
+  return 64 * small::split_of(nf, f64, one).nw;
 }
 
 // One tile (256 lines, one per lane) of a c2c pass of length NF (= P.n).
@@ -386,13 +386,13 @@ __device__ __forceinline__ void small_line_tile(const pfk_pass &P, const int64_t
 // Steps 1-3 of the split form on the lane's registers: x[m] = line[j + G*m]
 // (m < M) in, x[i*G + k2] = X[j + G*i + M*k2] out.  buf holds the tile's
 // rows (row r of RP slots); the exchange ends with a barrier.
-template <typename T, int NF>
+template <typename T, int NF, bool ONE = false>
 __device__ __forceinline__ void split_fft(small::C2<T> *x, small::C2<T> *buf, const int r,
                                           const int j, const T sg,
                                           const small::C2<T> *__restrict__ tw) {
   using namespace small;
   using CT = C2<T>;
-  constexpr split_shape SH = split_of(NF, sizeof(T) == 8);
+  constexpr split_shape SH = split_of(NF, sizeof(T) == 8, ONE);
   constexpr int G = SH.g, M = SH.m, NI = SH.ni, IB = SH.ib, NCH = SH.nch, RP = SH.rp;
   constexpr bool EXACT = M % G == 0;
   // ---------------- step 1: M-point DFT of the lane's sequence ----------------
@@ -660,7 +660,7 @@ __device__ __forceinline__ void small_split_real_tile(const pfk_pass &P, const i
                                                       T *__restrict__ dst, unsigned char *smem) {
   using namespace small;
   using CT = C2<T>;
-  constexpr split_shape SH = split_of(NF, sizeof(T) == 8);
+  constexpr split_shape SH = split_of(NF, sizeof(T) == 8, true);
   constexpr int G = SH.g, M = SH.m, NI = SH.ni, LW = SH.lw, LPT = SH.lpt, NW = SH.nw,
                 RP = SH.rp;
   constexpr int NT = NW * 64;
@@ -763,7 +763,7 @@ __device__ __forceinline__ void small_split_real_tile(const pfk_pass &P, const i
   }
   lds_barrier();   // the exchange reuses the rows
 
-  split_fft<T, NF>(x, buf, r, j, (T)P.sign, reinterpret_cast<const CT *>(P.tw));
+  split_fft<T, NF, true>(x, buf, r, j, (T)P.sign, reinterpret_cast<const CT *>(P.tw));
 
 #pragma unroll
   for (int i = 0; i < NI; ++i)
@@ -800,10 +800,9 @@ __device__ __forceinline__ void small_split_real_tile(const pfk_pass &P, const i
 }
 
 // Whether the split form computes even-length real passes of length NF
-// (nf = NF, n = 2*NF): a split shape of one chunk.
+// (nf = NF, n = 2*NF): every split shape, taken in one chunk.
 __host__ __device__ constexpr bool split_real_length(int nf, bool f64) {
-  const small::split_shape s = small::split_of(nf, f64);
-  return s.g > 1 && s.nch == 1 && s.wc == nf;
+  return small::split_of(nf, f64, true).g > 1;
 }
 
 // One tile of a c2c pass of length NF: small_lines_per_tile(NF) lines.

@@ -1,6 +1,7 @@
 """GPU parity of even-length r2c / c2r passes computed by the split form of
 the short-length kernel (the half-length complex line of length nf = n/2
-for nf in the short and mid length lists whose split needs one LDS chunk)
+for nf in the short (above 16) and mid length lists, split in one LDS
+chunk)
 against the oracle: real lines contiguous and strided (transposed layouts),
 embed / trunc and shifts on the real dimension, both precisions, and the
 adjoint passes (Hermitian weights) through dot-product identities.  Plans
@@ -17,7 +18,7 @@ pytestmark = pytest.mark.gpu
 TO, TI, SI, SO, EST = (P.TRANSPOSED_OUT, P.TRANSPOSED_IN, P.SHIFTED_IN, P.SHIFTED_OUT,
                        P.ESTIMATE)
 F32 = [18, 20, 24, 30, 32, 36, 48, 50, 60, 72, 90, 96, 100, 120, 144, 240, 480, 960]
-F64 = [18, 30, 36, 54, 60, 72, 96, 108, 144, 240, 480, 960]
+F64 = [18, 20, 24, 30, 32, 36, 48, 54, 60, 72, 96, 108, 144, 240, 480, 960]
 
 
 def tol(prec):
