@@ -67,9 +67,11 @@ k_batch_small(const pfk_task *__restrict__ tasks, const int t0, const int t1,
 }
 
 // short lengths the batch runs with the short-length kernel: those whose
-// workgroups have 4 waves (the batch launches 256 threads)
+// workgroups have 4 waves (the batch launches 256 threads), odd lengths
+// above 16 excepted (one line per lane holds too many registers inside the
+// work-list loop)
 __host__ __device__ constexpr bool batch_small_length(int nf, bool f64) {
-  return small_length(nf, f64) && small_threads(nf, f64) == 256;
+  return small_length(nf, f64) && small_threads(nf, f64) == 256 && !(nf > 16 && (nf & 1));
 }
 
 // kernel of a task group: 0 = general, else the short length NF
@@ -90,7 +92,7 @@ static const void *batch_kernel(int cls) {
     PFK_BATCH_SMALL(120) PFK_BATCH_SMALL(128)
     PFK_BATCH_SMALL(9) PFK_BATCH_SMALL(18) PFK_BATCH_SMALL(36) PFK_BATCH_SMALL(50)
     PFK_BATCH_SMALL(54) PFK_BATCH_SMALL(72) PFK_BATCH_SMALL(90) PFK_BATCH_SMALL(100)
-    PFK_BATCH_SMALL(108)
+    PFK_BATCH_SMALL(108) PFK_BATCH_SMALL(25) PFK_BATCH_SMALL(27) PFK_BATCH_SMALL(45)
 #undef PFK_BATCH_SMALL
     default: break;
   }

@@ -90,7 +90,7 @@ static int launch_small_t(const pfk_pass *p, const void *src, void *dst) {
     PFK_SMALL_LAUNCH(120) PFK_SMALL_LAUNCH(128)
     PFK_SMALL_LAUNCH(9) PFK_SMALL_LAUNCH(18) PFK_SMALL_LAUNCH(36) PFK_SMALL_LAUNCH(50)
     PFK_SMALL_LAUNCH(54) PFK_SMALL_LAUNCH(72) PFK_SMALL_LAUNCH(90) PFK_SMALL_LAUNCH(100)
-    PFK_SMALL_LAUNCH(108)
+    PFK_SMALL_LAUNCH(108) PFK_SMALL_LAUNCH(25) PFK_SMALL_LAUNCH(27) PFK_SMALL_LAUNCH(45)
     PFK_SMALL_LAUNCH(144) PFK_SMALL_LAUNCH(160) PFK_SMALL_LAUNCH(192) PFK_SMALL_LAUNCH(200)
     PFK_SMALL_LAUNCH(216) PFK_SMALL_LAUNCH(240) PFK_SMALL_LAUNCH(288) PFK_SMALL_LAUNCH(320)
     PFK_SMALL_LAUNCH(384) PFK_SMALL_LAUNCH(400) PFK_SMALL_LAUNCH(432) PFK_SMALL_LAUNCH(480)

@@ -817,8 +817,8 @@ __device__ __forceinline__ void small_tile(const pfk_pass &P, const int64_t tile
 }
 
 // Lengths with a small_tile instantiation: 2^a 3^b 5^c up to 128 except
-// the odd ones above 16 (one line per lane would not fit the registers),
-// and 90 (45 values per lane) in complex64 only.
+// the odd ones above 27 (one line per lane would not fit the registers;
+// 45 and 90, with 45 values per lane, in complex64 only).
 __host__ __device__ constexpr bool small_length(int nf, bool f64) {
   switch (nf) {
     case 2: case 3: case 4: case 5: case 6: case 8: case 9: case 10: case 12: case 15:
@@ -826,7 +826,9 @@ __host__ __device__ constexpr bool small_length(int nf, bool f64) {
     case 50: case 54: case 60: case 64: case 72: case 80: case 96: case 100: case 108:
     case 120: case 128:
       return true;
-    case 90:
+    case 25: case 27:
+      return true;
+    case 45: case 90:
       return !f64;
     default:
       return false;
@@ -863,7 +865,7 @@ __device__ __forceinline__ void small_dispatch(const pfk_pass &P, const int64_t 
     PFK_SMALL_CASE(128)
     PFK_SMALL_CASE(9) PFK_SMALL_CASE(18) PFK_SMALL_CASE(36) PFK_SMALL_CASE(50)
     PFK_SMALL_CASE(54) PFK_SMALL_CASE(72) PFK_SMALL_CASE(90) PFK_SMALL_CASE(100)
-    PFK_SMALL_CASE(108)
+    PFK_SMALL_CASE(108) PFK_SMALL_CASE(25) PFK_SMALL_CASE(27) PFK_SMALL_CASE(45)
 #undef PFK_SMALL_CASE
     default: break;
   }

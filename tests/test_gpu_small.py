@@ -17,8 +17,8 @@ from gpu_util import spec_of, random_input, run_gpu, oracle_out, rel_l2, comm_fo
 pytestmark = pytest.mark.gpu
 
 SI, SO, TO = P.SHIFTED_IN, P.SHIFTED_OUT, P.TRANSPOSED_OUT
-LENGTHS = [2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 30, 32, 36, 40, 48, 50, 54, 60, 64,
-           72, 80, 90, 96, 100, 108, 120, 128]
+LENGTHS = [2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25, 27, 30, 32, 36, 40, 45, 48, 50,
+           54, 60, 64, 72, 80, 90, 96, 100, 108, 120, 128]
 
 
 def tol(prec):
@@ -46,7 +46,7 @@ def run_batch(spec, x, prec):
 def cases():
     for L in LENGTHS:
         for prec in ("f32", "f64"):
-            if not (L == 90 and prec == "f64"):   # 45 values per lane: complex64 only
+            if not (L in (45, 90) and prec == "f64"):   # 45 values per lane: complex64 only
                 yield L, prec
 
 
