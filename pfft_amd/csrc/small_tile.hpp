@@ -835,10 +835,11 @@ __host__ __device__ constexpr bool small_length(int nf, bool f64) {
   }
 }
 
-// Lengths above 128 with a split-form instantiation for the per-pass launch:
-// 2^a 3^b 5^c, not powers of two, whose split holds at most 32 complex64
-// (16 complex128) values per lane.
+// Lengths above 128 with a split-form instantiation for the per-pass launch
+// (both precisions): 2^a 3^b 5^c, not powers of two, whose split holds at
+// most 32 values per lane in either step.
 __host__ __device__ constexpr bool mid_length(int nf, bool f64) {
+  (void)f64;
   switch (nf) {
     case 144: case 160: case 192: case 240:
     case 200: case 216: case 288: case 320: case 384: case 400: case 432: case 480:
