@@ -10,8 +10,11 @@
 // b + grid, b + 2*grid, ...).  There are no waits and no atomics inside a
 // launch; the order between levels (pass p+1 of a plan reads what pass p
 // wrote) is given by the launch boundaries.  All launches are captured into
-// one hipGraph and replayed.  Separate kernels per group keep each
-// kernel's register allocation (and so its occupancy) its own.
+// one hipGraph and replayed; within a level the groups are captured as
+// parallel branches (forked over four streams, joined before the next
+// level), so one group's tail overlaps the next group's start.  Separate
+// kernels per group keep each kernel's register allocation (and so its
+// occupancy) its own.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -110,6 +113,7 @@ typedef struct {
   int t0, t1;
   int64_t i0, i1;
   int grid;
+  int level;
   size_t lds;
 } batch_group;
 
@@ -123,21 +127,72 @@ struct pfk_batch {
   hipGraphExec_t exec;
 };
 
-static int batch_launch(pfk_batch *b) {
-  for (int k = 0; k < b->ngroups; k++) {
-    batch_group *g = &b->groups[k];
-    const int64_t items = g->i1 - g->i0;
-    if (items <= 0) continue;
-    const int grid = (int)(items < g->grid ? items : g->grid);
-    const pfk_task *tasks = b->d_tasks;
-    int t0 = g->t0, t1 = g->t1;
-    int64_t i0 = g->i0, i1 = g->i1;
-    void *args[] = {(void *)&tasks, (void *)&t0, (void *)&t1, (void *)&i0, (void *)&i1};
-    hipError_t e = hipLaunchKernel(g->kernel, dim3(grid), dim3(256), args, g->lds, g_stream);
-    if (e == hipSuccess) e = hipGetLastError();
-    if (e != hipSuccess) { set_error("pfk_batch launch", e); return 1; }
-  }
+static int launch_group(const pfk_batch *b, const batch_group *g, hipStream_t s) {
+  const int64_t items = g->i1 - g->i0;
+  if (items <= 0) return 0;
+  const int grid = (int)(items < g->grid ? items : g->grid);
+  const pfk_task *tasks = b->d_tasks;
+  int t0 = g->t0, t1 = g->t1;
+  int64_t i0 = g->i0, i1 = g->i1;
+  void *args[] = {(void *)&tasks, (void *)&t0, (void *)&t1, (void *)&i0, (void *)&i1};
+  hipError_t e = hipLaunchKernel(g->kernel, dim3(grid), dim3(256), args, g->lds, s);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) { set_error("pfk_batch launch", e); return 1; }
   return 0;
+}
+
+static int batch_launch(pfk_batch *b) {
+  for (int k = 0; k < b->ngroups; k++)
+    if (launch_group(b, &b->groups[k], g_stream)) return 1;
+  return 0;
+}
+
+// Under graph capture: the groups of one level are independent, so they
+// fork over the library stream and three auxiliary streams (default
+// priority) and join before the next level; the graph then holds the
+// level's launches as parallel branches.
+#define PFK_BATCH_AUX 3
+static hipStream_t g_aux[PFK_BATCH_AUX];
+
+static int batch_capture_forked(pfk_batch *b) {
+  for (int i = 0; i < PFK_BATCH_AUX; i++)
+    if (!g_aux[i] && hipStreamCreateWithFlags(&g_aux[i], hipStreamNonBlocking) != hipSuccess) {
+      g_aux[i] = nullptr;
+      set_error_msg("pfk_batch: auxiliary stream creation failed");
+      return 1;
+    }
+  hipEvent_t ev[PFK_MAX_GROUPS + PFK_MAXSTAGES];
+  int nev = 0, err = 0;
+  int k = 0;
+  while (k < b->ngroups && !err) {
+    int kend = k;
+    while (kend < b->ngroups && b->groups[kend].level == b->groups[k].level) kend++;
+    hipEvent_t start;
+    if (hipEventCreateWithFlags(&start, hipEventDisableTiming) != hipSuccess) { err = 1; break; }
+    ev[nev++] = start;
+    if (hipEventRecord(start, g_stream) != hipSuccess) { err = 1; break; }
+    hipEvent_t done[PFK_MAX_GROUPS];
+    int ndone = 0;
+    for (int i = k; i < kend && !err; i++) {
+      const int slot = (i - k) % (PFK_BATCH_AUX + 1);
+      hipStream_t s = slot == 0 ? g_stream : g_aux[slot - 1];
+      if (s != g_stream && hipStreamWaitEvent(s, start, 0) != hipSuccess) { err = 1; break; }
+      if (launch_group(b, &b->groups[i], s)) { err = 1; break; }
+      if (s != g_stream && i + PFK_BATCH_AUX + 1 >= kend) {   // last group of this stream
+        hipEvent_t d;
+        if (hipEventCreateWithFlags(&d, hipEventDisableTiming) != hipSuccess) { err = 1; break; }
+        ev[nev++] = d;
+        done[ndone++] = d;
+        if (hipEventRecord(d, s) != hipSuccess) { err = 1; break; }
+      }
+    }
+    for (int i = 0; i < ndone && !err; i++)
+      if (hipStreamWaitEvent(g_stream, done[i], 0) != hipSuccess) err = 1;
+    k = kend;
+  }
+  for (int i = 0; i < nev; i++) (void)hipEventDestroy(ev[i]);
+  if (err) set_error_msg("pfk_batch: forked capture failed");
+  return err;
 }
 
 static int task_class(const pfk_task *t) {
@@ -195,6 +250,7 @@ extern "C" pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_grap
         return nullptr;
       }
       batch_group *g = &b->groups[b->ngroups++];
+      g->level = l;
       g->t0 = i;
       g->i0 = 0;
       int64_t item = 0;
@@ -241,7 +297,7 @@ extern "C" pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_grap
   if (use_graph) {
     e = hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      const int r = batch_launch(b);
+      const int r = batch_capture_forked(b);
       hipError_t e2 = hipStreamEndCapture(g_stream, &b->graph);
       if (r || e2 != hipSuccess) e = (e2 != hipSuccess) ? e2 : hipErrorUnknown;
     }
