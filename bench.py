@@ -68,6 +68,32 @@ def traffic_from_profiles(plan, npass):
     return None
 
 
+def hbm_probe(L, bytes_per_buffer=1 << 30, iters=3):
+    """Attainable HBM bandwidth on this GPU: a streaming kernel swept over
+    read-only / write-only / copy, 16 / 32 / 64 bytes per lane and access,
+    cached or nontemporal accesses, and five grids (persistent grids of 2, 4,
+    8 and 16 workgroups per CU, and one workgroup per chunk).  Returns the
+    best configuration of each mix (GB/s of bytes read plus written)."""
+    L.pfft_amd_bw_probe.restype = ctypes.c_double
+    L.pfft_amd_bw_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_longlong,
+                                    ctypes.c_size_t, ctypes.c_int]
+    import torch
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    out = {}
+    for mode, name in ((0, "read"), (1, "write"), (2, "copy")):
+        best = None
+        for width in (16, 32, 64):
+            for nt in (0, 1):
+                for grid in (2 * ncu, 4 * ncu, 8 * ncu, 16 * ncu, 0):
+                    g = L.pfft_amd_bw_probe(mode, width, nt, grid, bytes_per_buffer, iters)
+                    if g > 0 and (best is None or g > best["GBps"]):
+                        best = {"GBps": round(g, 1), "bytes_per_lane": width,
+                                "nontemporal": bool(nt),
+                                "grid": "one pass" if grid == 0 else "%d per CU" % (grid // ncu)}
+        out[name] = best
+    return out
+
+
 def cpu_baseline(n, cores, seed):
     from oracle import semantics as S
     S.WORKERS = cores
@@ -109,8 +135,6 @@ def main():
                                        ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]
     L.pfft_amd_plan_npasses.argtypes = [ctypes.c_void_p]
-    L.pfft_amd_copy_bandwidth.restype = ctypes.c_double
-    L.pfft_amd_copy_bandwidth.argtypes = [ctypes.c_size_t, ctypes.c_int]
     L.pfft_amd_describe_plan.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
 
     n = args.n
@@ -155,7 +179,8 @@ def main():
             passes.append({"plan": name, "pass": k, "ms": msv[k], "bytes": byv[k],
                            "GBps": byv[k] / (msv[k] * 1e-3) / 1e9, "desc": lines[k]})
     dom = max(passes, key=lambda p: p["ms"])
-    copy_gbs = L.pfft_amd_copy_bandwidth(1 << 30, 10)
+    probe = hbm_probe(L)
+    copy_gbs = probe["copy"]["GBps"]
     sum_ms = sum(p["ms"] for p in passes)
     bytes_pair = sum(p["bytes"] for p in passes)
 
@@ -196,6 +221,7 @@ def main():
             "pipeline_GBps_alg": round(bytes_pair / (sum_ms * 1e-3) / 1e9, 1),
             "frac_of_copy_bw": round(bytes_pair / (sum_ms * 1e-3) / 1e9 / copy_gbs, 4),
             "measured_copy_GBps": round(copy_gbs, 1),
+            "hbm_probe": probe,
             "passes": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()}
                        for p in passes],
         },

@@ -415,12 +415,18 @@ int pfft_amd_memcpy_to_host(void *dst_host, const void *src_dev, size_t bytes);
 /* Measurement hooks: number of device passes of a plan; mean duration
  * (ms) of each pass over `iters` runs on device arrays in/out together
  * with the bytes each pass reads plus writes; a text description of the
- * passes; the device's streaming-copy bandwidth in GB/s. */
+ * passes. */
 int pfft_amd_plan_npasses(const void *plan);
 int pfft_amd_time_passes(const void *plan, void *in, void *out, int iters, double *ms,
                          double *bytes);
 int pfft_amd_describe_plan(const void *plan, char *buf, size_t len);
-double pfft_amd_copy_bandwidth(size_t bytes, int iters);
+/* Attainable HBM bandwidth: the best of `iters` timed launches of a
+ * streaming kernel over `bytes` per buffer, in GB/s of bytes read plus
+ * written.  mode 0 read only, 1 write only, 2 copy; width 16, 32 or 64
+ * contiguous bytes per lane and access; nt nonzero for nontemporal
+ * accesses; grid = 256-thread workgroups of a persistent grid, or 0 for
+ * one workgroup per 4 KiB..16 KiB chunk.  Negative on error. */
+double pfft_amd_bw_probe(int mode, int width, int nt, long long grid, size_t bytes, int iters);
 /* Wisdom: the launch variants the planner chose by timing (flags
  * PFFT_MEASURE, PFFT_PATIENT, PFFT_EXHAUSTIVE; none with PFFT_ESTIMATE),
  * keyed by pass geometry.  Export writes the process-wide cache to a text

@@ -177,7 +177,9 @@ int pfft_amd_describe_plan(const void *plan, char *buf, size_t len) {
   if (!plan || !buf || !len) return 1;
   return pa_plan_describe((const pa_plan *)plan, buf, len);
 }
-double pfft_amd_copy_bandwidth(size_t bytes, int iters) { return pfk_copy_bandwidth(bytes, iters); }
+double pfft_amd_bw_probe(int mode, int width, int nt, long long grid, size_t bytes, int iters) {
+  return pfk_bw_probe(mode, width, nt, grid, bytes, iters);
+}
 
 int pfft_amd_export_wisdom(const char *filename) { return pa_wisdom_export(filename); }
 int pfft_amd_import_wisdom(const char *filename) { return pa_wisdom_import(filename); }

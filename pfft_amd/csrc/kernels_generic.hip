@@ -935,51 +935,3 @@ int pfk_scale(void *data, int prec, int64_t nreal, double factor) {
 
 }  // extern "C"
 
-namespace pfk {
-// Streaming copy used to measure the attainable HBM bandwidth: each thread
-// moves 4 independent 16-byte vectors per iteration.
-__global__ void __launch_bounds__(256) k_copy16(const double2 *__restrict__ a,
-                                                double2 *__restrict__ b, int64_t n) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const double2 x0 = a[i], x1 = a[i + stride], x2 = a[i + 2 * stride], x3 = a[i + 3 * stride];
-    b[i] = x0;
-    b[i + stride] = x1;
-    b[i + 2 * stride] = x2;
-    b[i + 3 * stride] = x3;
-  }
-  for (; i < n; i += stride) b[i] = a[i];
-}
-}  // namespace pfk
-
-extern "C" double pfk_copy_bandwidth(size_t bytes, int iters) {
-  if (pfk_init()) return -1;
-  const int64_t n = (int64_t)(bytes / 16);
-  double2 *a = nullptr, *b = nullptr;
-  if (hipMalloc(&a, n * 16) != hipSuccess) return -1;
-  if (hipMalloc(&b, n * 16) != hipSuccess) { (void)hipFree(a); return -1; }
-  (void)hipMemsetAsync(a, 0, n * 16, g_stream);
-  (void)hipMemsetAsync(b, 0, n * 16, g_stream);
-  hipEvent_t e0, e1;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
-  double best = 0;
-  const int grids[3] = {256 * 8, 256 * 32, 256 * 128};
-  for (int g = 0; g < 3; g++)
-    for (int it = 0; it < iters + 1; it++) {
-      (void)hipEventRecord(e0, g_stream);
-      hipLaunchKernelGGL(k_copy16, dim3(grids[g]), dim3(256), 0, g_stream, a, b, n);
-      (void)hipEventRecord(e1, g_stream);
-      (void)hipEventSynchronize(e1);
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      const double gbs = 2.0 * n * 16 / (ms * 1e-3) / 1e9;
-      if (it > 0 && gbs > best) best = gbs;
-    }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(a);
-  (void)hipFree(b);
-  return best;
-}

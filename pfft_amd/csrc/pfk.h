@@ -281,9 +281,10 @@ int  pfk_event_destroy(void *ev);
 /* Complex conjugation / scaling utility kernels on device arrays. */
 int  pfk_scale(void *data, int prec, int64_t nreal, double factor);
 
-/* Streaming-copy bandwidth of the device (GB/s, read+write bytes counted)
- * over a buffer of `bytes`, best of `iters` timed copies; <0 on error. */
-double pfk_copy_bandwidth(size_t bytes, int iters);
+/* Streaming bandwidth probe (GB/s of bytes read plus written), best of
+ * `iters` launches: mode 0 read, 1 write, 2 copy; width 16/32/64 bytes per
+ * lane and access; nt nontemporal; grid workgroups (0: one pass). */
+double pfk_bw_probe(int mode, int width, int nt, int64_t grid, size_t bytes, int iters);
 
 #ifdef __cplusplus
 }
