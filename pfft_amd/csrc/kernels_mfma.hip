@@ -666,9 +666,13 @@ int mfma_covers(const pfk_pass *p) {
   return p->nf >= 256 && p->nf <= 4096 && (p->nf & (p->nf - 1)) == 0;
 }
 
-// 0 launched, 1 not covered, -1 error
+// 0 launched, 1 not covered, -1 error.  Plain passes of 256..1024 points
+// run the chained kernels (kernels_mfma_chain.hip); embedded, truncated or
+// shifted passes the register kernel above.
 int launch_mfma(const pfk_pass *p, const void *src, void *dst) {
   if (!mfma_covers(p)) return 1;
+  const int rc = launch_mfma_chain(p, src, dst);
+  if (rc != 1) return rc;
   const int mode = p->mfma - 1;
   switch (p->nf) {
     case 256: return mode ? launch_mfma_reg<256, 1>(p, src, dst) : launch_mfma_reg<256, 0>(p, src, dst);

@@ -336,12 +336,31 @@ __device__ __forceinline__ void load_c2c(const pfk_pass &P, const T *src, const 
   }
 }
 
+// c2c loads of a plain pass (no embed, no modulation): one 64-bit address
+// per thread, then constant strides (TT*sin, uniform) between its elements
+template <typename T, int NF, int C, int E>
+__device__ __forceinline__ void load_c2c_pl(const pfk_pass &P, const T *src, const TileCtx &tc,
+                                            const bool live, const int c, const int t,
+                                            C2<T> *v) {
+  constexpr int TT = NF / E;
+  const C2<T> *s0 = reinterpret_cast<const C2<T> *>(src) + tc.sb;
+  const C2<T> *s = (live && c < tc.nl) ? s0 + (int64_t)c * P.tsin + (int64_t)t * P.sin : s0;
+  const int64_t st = live ? (int64_t)TT * P.sin : 0;
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = ld_stream(s + m * st);
+}
+
 // LM: line-major lane order (thread = c*T + t: a wave's lanes run along a
 // line), for passes whose lines are contiguous in memory; otherwise
 // thread = t*C + c (a wave's lanes run across the tile's lines).
 // TW: output k of tile line c0 + c is multiplied by W_twn^((c0+c)*k) from
 // the two-level table tw2 (first step of a long line, pfk.h).
-template <typename T, int NF, int C, int E, int SIGN, bool LM = false, bool TW = false>
+// PL: plain pass (every input and output position present, no
+// modulation): no masks or selects, and element addresses advance by a
+// uniform stride, which halves the vector instructions spent per element
+// outside the butterflies.
+template <typename T, int NF, int C, int E, int SIGN, bool LM = false, bool TW = false,
+          bool PL = false>
 __global__ void __launch_bounds__(C *NF / E)
 k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
            const T *__restrict__ src, T *__restrict__ dst) {
@@ -368,17 +387,28 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
     const int64_t ni = item + gridDim.x;
     const bool more = ni < item_end;
     const TileCtx tn = tile_at(P, C, more ? ni : item, ntiles);
-    load_c2c<T, NF, C, E>(P, src, tn, more, c, t, nxt);
+    if constexpr (PL) load_c2c_pl<T, NF, C, E>(P, src, tn, more, c, t, nxt);
+    else load_c2c<T, NF, C, E>(P, src, tn, more, c, t, nxt);
     const bool valid = c < tc.nl;
+    if constexpr (!PL) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int j = t + TT * m - zin;
-      const bool ok = valid && j >= 0 && j < nin;
-      const T f = msign<T>(P.imod, j);
-      cur[m] = ok ? C2<T>{cur[m].x * f, cur[m].y * f} : C2<T>{0, 0};
+      for (int m = 0; m < E; ++m) {
+        const int j = t + TT * m - zin;
+        const bool ok = valid && j >= 0 && j < nin;
+        const T f = msign<T>(P.imod, j);
+        cur[m] = ok ? C2<T>{cur[m].x * f, cur[m].y * f} : C2<T>{0, 0};
+      }
     }
     fft_regs<T, NF, C, E, SIGN, LM>(cur, lds, t, c, tw);
-    if (valid) {
+    if (PL && valid) {
+      // lines are independent, so a ragged tile's unused lines (garbage in,
+      // never stored) cannot affect the stored ones
+      C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout +
+                 (int64_t)t * P.sout;
+      const int64_t st = (int64_t)TT * P.sout;
+#pragma unroll
+      for (int m = 0; m < E; ++m) st_stream(d + m * st, cur[m]);
+    } else if (valid) {
       C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -402,7 +432,8 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
     return true;
   };
   C2<T> va[E], vb[E];
-  load_c2c<T, NF, C, E>(P, src, tc, true, c0, t0, va);
+  if constexpr (PL) load_c2c_pl<T, NF, C, E>(P, src, tc, true, c0, t0, va);
+  else load_c2c<T, NF, C, E>(P, src, tc, true, c0, t0, va);
   if constexpr (TWL) lds_barrier();
   while (step(va, vb) && step(vb, va)) {
   }
@@ -424,7 +455,7 @@ __device__ __forceinline__ void load_r2c(const pfk_pass &P, const T *src, const 
   for (int m = 0; m < E; ++m) v[m] = ld_stream(s + (live ? t : 0) + step * m);
 }
 
-template <typename T, int NF, int C, int E, int SIGN>
+template <typename T, int NF, int C, int E, int SIGN, bool PL = false>
 __global__ void __launch_bounds__(C *NF / E)
 k_r2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
            const T *__restrict__ src, T *__restrict__ dst) {
@@ -454,18 +485,37 @@ k_r2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
     const TileCtx tn = tile_at(P, C, more ? ni : item, ntiles);
     load_r2c<T, NF, C, E>(P, src, tn, more, c, t, nxt);
     const bool valid = c < tc.nl;
+    if constexpr (!PL) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int p = t + TT * m;
-      const T fx = msign<T>(P.imod, 2 * p), fy = msign<T>(P.imod, 2 * p + 1);
-      cur[m] = valid ? C2<T>{cur[m].x * fx, cur[m].y * fy} : C2<T>{0, 0};
+      for (int m = 0; m < E; ++m) {
+        const int p = t + TT * m;
+        const T fx = msign<T>(P.imod, 2 * p), fy = msign<T>(P.imod, 2 * p + 1);
+        cur[m] = valid ? C2<T>{cur[m].x * fx, cur[m].y * fy} : C2<T>{0, 0};
+      }
     }
     fft_regs<T, NF, C, E, SIGN>(cur, lds, t, c, tw);
     // mirror exchange: Z_{h-k}
 #pragma unroll
     for (int m = 0; m < E; ++m) lds[(t + TT * m) * C + c] = cur[m];
     lds_barrier();
-    if (valid) {
+    if (PL && valid) {
+      // every output k = t + T*m (and the Nyquist bin, thread t = 0) is kept
+      C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
+      C2<T> *dk = d + (int64_t)t * P.sout;
+      const int64_t st = (int64_t)TT * P.sout;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + TT * m;
+        const C2<T> a = cur[m];
+        const C2<T> b = lds[((H - k) & (H - 1)) * C + c];
+        const C2<T> Ev{(a.x + b.x) * (T)0.5, (a.y - b.y) * (T)0.5};
+        const C2<T> O{(a.y + b.y) * (T)0.5, -(a.x - b.x) * (T)0.5};
+        const C2<T> WO = cmul(O, rtw[k]);
+        st_stream(dk + m * st, C2<T>{Ev.x + WO.x, Ev.y + WO.y});
+        if (m == 0 && t == 0)
+          st_stream(d + (int64_t)H * P.sout, C2<T>{Ev.x - O.x, Ev.y - O.y});
+      }
+    } else if (valid) {
       C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -519,7 +569,19 @@ __device__ __forceinline__ void load_c2r(const pfk_pass &P, const T *src, const 
   xh = s[(int64_t)((live && okh) ? jh : 0) * P.sin];
 }
 
-template <typename T, int NF, int C, int E, int SIGN>
+// plain half spectrum (all n/2+1 bins, no modulation): loads with uniform
+// strides
+template <typename T, int NF, int C, int E>
+__device__ __forceinline__ void load_c2r_pl(const pfk_pass &P, const T *src, const TileCtx &tc,
+                                            const bool live, const int c, const int t, C2<T> *v,
+                                            C2<T> &xh) {
+  load_c2c_pl<T, NF, C, E>(P, src, tc, live, c, t, v);
+  const C2<T> *s0 = reinterpret_cast<const C2<T> *>(src) + tc.sb;
+  const C2<T> *s = (live && c < tc.nl) ? s0 + (int64_t)c * P.tsin : s0;
+  xh = ld_stream(s + (live ? (int64_t)NF * P.sin : 0));
+}
+
+template <typename T, int NF, int C, int E, int SIGN, bool PL = false>
 __global__ void __launch_bounds__(C *NF / E)
 k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
            const T *__restrict__ src, T *__restrict__ dst) {
@@ -547,23 +609,30 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
     const int64_t ni = item + gridDim.x;
     const bool more = ni < item_end;
     const TileCtx tn = tile_at(P, C, more ? ni : item, ntiles);
-    load_c2r<T, NF, C, E>(P, src, tn, more, c, t, nxt, nxth);
+    if constexpr (PL) load_c2r_pl<T, NF, C, E>(P, src, tn, more, c, t, nxt, nxth);
+    else load_c2r<T, NF, C, E>(P, src, tn, more, c, t, nxt, nxth);
     const bool valid = c < tc.nl;
     // X_k, k = t + T*m, and X_h (kept by thread t == 0) into LDS
+    if constexpr (PL) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int k = t + TT * m;
-      const int j = k - zin;
-      const bool ok = valid && j >= 0 && j < nin;
-      T f = msign<T>(P.imod, j);
-      if (P.hw == 1 && k != 0) f *= (T)0.5;
-      cur[m] = ok ? C2<T>{cur[m].x * f, cur[m].y * f} : C2<T>{0, 0};
-      lds[k * C + c] = cur[m];
-    }
-    if (t == 0) {
-      const int jh = H - zin;
-      const bool okh = valid && jh >= 0 && jh < nin;
-      lds[H * C + c] = okh ? C2<T>{curh.x * msign<T>(P.imod, jh), (T)0} : C2<T>{0, 0};
+      for (int m = 0; m < E; ++m) lds[(t + TT * m) * C + c] = cur[m];
+      if (t == 0) lds[H * C + c] = C2<T>{curh.x, (T)0};
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + TT * m;
+        const int j = k - zin;
+        const bool ok = valid && j >= 0 && j < nin;
+        T f = msign<T>(P.imod, j);
+        if (P.hw == 1 && k != 0) f *= (T)0.5;
+        cur[m] = ok ? C2<T>{cur[m].x * f, cur[m].y * f} : C2<T>{0, 0};
+        lds[k * C + c] = cur[m];
+      }
+      if (t == 0) {
+        const int jh = H - zin;
+        const bool okh = valid && jh >= 0 && jh < nin;
+        lds[H * C + c] = okh ? C2<T>{curh.x * msign<T>(P.imod, jh), (T)0} : C2<T>{0, 0};
+      }
     }
     lds_barrier();
     // Z_k = E_k + i O_k
@@ -591,7 +660,7 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       for (int m = 0; m < E; ++m) {
         const int p = t + TT * m;
         C2<T> x = cur[m];
-        if (P.omod.on) {
+        if (!PL && P.omod.on) {
           x.x *= msign<T>(P.omod, 2 * p);
           x.y *= msign<T>(P.omod, 2 * p + 1);
         }
@@ -605,7 +674,8 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
     return true;
   };
   C2<T> va[E], vb[E], ha, hb;
-  load_c2r<T, NF, C, E>(P, src, tc, true, c0, t0, va, ha);
+  if constexpr (PL) load_c2r_pl<T, NF, C, E>(P, src, tc, true, c0, t0, va, ha);
+  else load_c2r<T, NF, C, E>(P, src, tc, true, c0, t0, va, ha);
   if constexpr (TWL) lds_barrier();
   while (step(va, ha, vb, hb) && step(vb, hb, va, ha)) {
   }
@@ -649,15 +719,25 @@ static int launch_items(void (*k)(pfk_pass, int64_t, int64_t, int64_t, const T *
   return 0;
 }
 
-template <typename T, int NF, int C, int E, int SIGN, bool TWOK>
-static int launch_kind(const pfk_pass *p, int64_t ntiles, const void *src, void *dst) {
+// A pass with every input and output position present and no modulation
+// or adjoint weights runs the plain (PL) instantiations.
+static bool pass_plain(const pfk_pass *p) {
+  if (p->zin || p->zout || p->imod.on || p->omod.on || p->hw || p->twn) return false;
+  if (p->kind == PFK_C2C) return p->nin == p->nf && p->nout == p->nf;
+  if (p->kind == PFK_R2C) return p->nin == p->n && p->nout == p->nf + 1;
+  if (p->kind == PFK_C2R) return p->nin == p->nf + 1 && p->nout == p->n;
+  return false;
+}
+
+template <typename T, int NF, int C, int E, int SIGN, bool TWOK, bool PL>
+static int launch_kind_pl(const pfk_pass *p, int64_t ntiles, const void *src, void *dst) {
   constexpr int NT = C * NF / E;
   static int64_t grid_c2c = 0, grid_lm = 0, grid_r2c = 0, grid_c2r = 0, grid_tw = 0;
   const bool persistent = !(p->variant & 4);
   const T *s = (const T *)src;
   T *d = (T *)dst;
   if (p->kind == PFK_C2C && p->twn) {
-    if constexpr (TWOK) {
+    if constexpr (TWOK && !PL) {
       if (!grid_tw) grid_tw = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, false, true>, NT);
       return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, false, true>, p, ntiles, grid_tw,
                           persistent, NT, s, d);
@@ -667,19 +747,30 @@ static int launch_kind(const pfk_pass *p, int64_t ntiles, const void *src, void 
   if (p->kind == PFK_C2C) {
     if constexpr (sizeof(T) == 4) {   // line-major lane order: complex64 only
       if (p->variant & 8) {
-        if (!grid_lm) grid_lm = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, true>, NT);
-        return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, true>, p, ntiles, grid_lm, persistent,
-                            NT, s, d);
+        if (!grid_lm) grid_lm = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, true, false, PL>, NT);
+        return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, true, false, PL>, p, ntiles, grid_lm,
+                            persistent, NT, s, d);
       }
     }
-    if (!grid_c2c) grid_c2c = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN>, NT);
-    return launch_items(k_c2c_pow2<T, NF, C, E, SIGN>, p, ntiles, grid_c2c, persistent, NT, s, d);
+    if (!grid_c2c) grid_c2c = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, false, false, PL>, NT);
+    return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, false, false, PL>, p, ntiles, grid_c2c,
+                        persistent, NT, s, d);
   } else if (p->kind == PFK_R2C) {
-    if (!grid_r2c) grid_r2c = persistent_grid(k_r2c_pow2<T, NF, C, E, SIGN>, NT);
-    return launch_items(k_r2c_pow2<T, NF, C, E, SIGN>, p, ntiles, grid_r2c, persistent, NT, s, d);
+    if (!grid_r2c) grid_r2c = persistent_grid(k_r2c_pow2<T, NF, C, E, SIGN, PL>, NT);
+    return launch_items(k_r2c_pow2<T, NF, C, E, SIGN, PL>, p, ntiles, grid_r2c, persistent, NT,
+                        s, d);
   }
-  if (!grid_c2r) grid_c2r = persistent_grid(k_c2r_pow2<T, NF, C, E, SIGN>, NT);
-  return launch_items(k_c2r_pow2<T, NF, C, E, SIGN>, p, ntiles, grid_c2r, persistent, NT, s, d);
+  if (!grid_c2r) grid_c2r = persistent_grid(k_c2r_pow2<T, NF, C, E, SIGN, PL>, NT);
+  return launch_items(k_c2r_pow2<T, NF, C, E, SIGN, PL>, p, ntiles, grid_c2r, persistent, NT, s,
+                      d);
+}
+
+// the plain instantiations exist for the default tiles only
+template <typename T, int NF, int C, int E, int SIGN, bool TWOK>
+static int launch_kind(const pfk_pass *p, int64_t ntiles, const void *src, void *dst) {
+  if constexpr (TWOK)
+    if (pass_plain(p)) return launch_kind_pl<T, NF, C, E, SIGN, true, true>(p, ntiles, src, dst);
+  return launch_kind_pl<T, NF, C, E, SIGN, TWOK, false>(p, ntiles, src, dst);
 }
 
 // TWOK: the table also instantiates the twiddled first step of long lines

@@ -36,9 +36,11 @@ void set_error_msg(const char *msg);
 int launch_fast(const pfk_pass *p, const void *src, void *dst);
 int launch_generic(const pfk_pass *p, const void *src, void *dst);
 int launch_mfma(const pfk_pass *p, const void *src, void *dst);
+int launch_mfma_chain(const pfk_pass *p, const void *src, void *dst);
 int launch_small(const pfk_pass *p, const void *src, void *dst);
 int fast_covers(const pfk_pass *p);   // 1 if launch_fast has a kernel for the shape
 int mfma_covers(const pfk_pass *p);   // 1 if launch_mfma has a kernel for the shape
+int mfma_chain_covers(const pfk_pass *p);   // plain 256..1024-point passes (chained MFMA)
 int small_covers(const pfk_pass *p);  // 1 if launch_small has a kernel for the shape
 
 }  // namespace pfk
