@@ -9,8 +9,11 @@ HBM.  One step = one r2c + c2r pair.
 Prints one JSON line (rank 0, N=1): throughput in GFLOP/s counted as
 5*N*log2(N)*0.5 per r2c or c2r transform, the per-pair time, the roofline
 of the dominant kernel (algorithmic bytes / measured duration vs the
-8 TB/s HBM peak, HIP events on the library stream) and a CPU baseline
-(the oracle timed on the host cores).
+8 TB/s HBM peak, HIP events on the library stream) beside the attainable
+bandwidth a streaming probe measures on the same GPU, the precision
+comparison ("mixed": the 1024x1024x512 c2c study in FP64, FP32, FP32-MFMA
+and split-BF16 with errors against an FP64 reference) and a CPU baseline
+(labelled stand-ins timed on this GPU's share of the host cores).
 """
 import argparse
 import ctypes
@@ -36,6 +39,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the precision comparison (FP64 vs FP32-MFMA / split-BF16)")
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--workload", default="cfg2",
@@ -94,10 +99,36 @@ def hbm_probe(L, bytes_per_buffer=1 << 30, iters=3):
     return out
 
 
-def cpu_baseline(n, cores, seed):
+def host_cpu():
+    """(threads to use, description): this GPU's share of the host CPU
+    (OMP_NUM_THREADS when set, else the CPUs this process may run on) and
+    the CPU model."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
+    cores = max(1, min(share, avail))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, "%s; %d logical CPUs available, %d used (this GPU's share)" % (
+        model, avail, cores)
+
+
+def cpu_baseline(n, cores, cpu_desc, seed):
+    """The reference PFFT/FFTW-MPI is not installed on the GPU box (and
+    cannot be fetched), so two labelled stand-ins run on the same host
+    threads: the oracle (FP64 restatement of the reference's pipeline,
+    numpy + scipy.fft pocketfft) and plain scipy.fft rfftn/irfftn."""
+    import scipy.fft as sfft
     from oracle import semantics as S
     S.WORKERS = cores
     N = n ** 3
+    flops = 5.0 * N * math.log2(N)
     fw = S.PlanSpec("r2c", [n] * 3, flags=2)
     bw = S.PlanSpec("c2r", [n] * 3, sign=+1, flags=1)
     x = np.random.default_rng(seed).standard_normal(N)
@@ -106,17 +137,48 @@ def cpu_baseline(n, cores, seed):
     y = S.execute_flat(bw, k, precision=np.float64)
     t1 = time.perf_counter()
     err = float(np.max(np.abs(y / N - x)))
-    flops = 5.0 * N * math.log2(N)
+    xr = x.reshape(n, n, n)
+    t2 = time.perf_counter()
+    kk = sfft.rfftn(xr, workers=cores)
+    yy = sfft.irfftn(kk, s=xr.shape, workers=cores)
+    t3 = time.perf_counter()
+    err2 = float(np.max(np.abs(yy - xr)))
     return {
         "value": flops / (t1 - t0) / 1e9,
         "unit": "GFLOP/s",
         "cores": cores,
         "kind": "port",
+        "cpu": cpu_desc,
         "sample": ("one full %d^3 FP64 r2c(TRANSPOSED_OUT)+c2r(TRANSPOSED_IN) pair through the "
-                   "oracle (numpy + scipy.fft pocketfft, %d worker threads); %.2f s; "
-                   "roundtrip max err %.1e; the reference PFFT/FFTW-MPI is not installed "
-                   "(no network), so this is the labelled stand-in" % (n, cores, t1 - t0, err)),
+                   "oracle (the reference pipeline restated in numpy + scipy.fft pocketfft, %d "
+                   "threads); %.2f s; roundtrip max err %.1e.  The reference PFFT/FFTW-MPI is "
+                   "not installed on the box and there is no network, so this is a labelled "
+                   "stand-in" % (n, cores, t1 - t0, err)),
+        "scipy_fft": {"value": flops / (t3 - t2) / 1e9, "unit": "GFLOP/s", "cores": cores,
+                      "sample": "scipy.fft.rfftn + irfftn of the same %d^3 input (untransposed "
+                                "layout), %d workers; %.2f s; roundtrip max err %.1e; second "
+                                "labelled stand-in" % (n, cores, t3 - t2, err2)},
     }
+
+
+def mixed_summary(r):
+    """compact form of the precision comparison for the bench line"""
+    forms = []
+    for f in r["forms"]:
+        e = {k: f[k] for k in ("form", "ms_fwd_bwd", "speedup_vs_fp64", "rel_l2_fwd",
+                               "max_err_fwd", "roundtrip_rel_l2", "GBps_alg") if k in f}
+        if f.get("passes_fwd"):
+            e["passes_fwd_ms"] = [p["ms"] for p in f["passes_fwd"]]
+            e["kernels"] = [p["kernel"] for p in f["passes_fwd"]]
+        forms.append(e)
+    out = {"workload": r["workload"], "reference": r.get("reference"), "steps": r["steps"],
+           "forms": forms}
+    mf = [f for f in forms if f["form"] == "fp32 mfma radix-16"]
+    if mf and "speedup_vs_fp64" in mf[0] and "rel_l2_fwd" in mf[0]:
+        out["target"] = {"form": "fp32 mfma radix-16", "speedup_vs_fp64": mf[0]["speedup_vs_fp64"],
+                         "rel_l2_fwd": mf[0]["rel_l2_fwd"], "goal": ">= 2x at <= 1e-5",
+                         "met": mf[0]["speedup_vs_fp64"] >= 2.0 and mf[0]["rel_l2_fwd"] <= 1e-5}
+    return out
 
 
 def main():
@@ -231,9 +293,14 @@ def main():
         o.destroy()
     for b in (dx, dk, dy):
         b.free()
+    cores, cpu_desc = host_cpu()
+    if args.cpu_cores:
+        cores = args.cpu_cores
+    if not args.no_mixed:
+        import bench_configs
+        out["mixed"] = mixed_summary(bench_configs.cfg3_forms(args.seed, steps=5, cores=cores))
     if not args.no_cpu:
-        cores = args.cpu_cores or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(n, cores, args.seed)
+        out["cpu_baseline"] = cpu_baseline(n, cores, cpu_desc, args.seed)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

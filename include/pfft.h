@@ -11,8 +11,8 @@
  * (staged through device buffers; the PCIe-inclusive path).
  *
  * Two precisions are exported, as in the reference (api/pfft.h:562-564):
- *   pfft_   double        complex = double[2]
- *   pfftf_  float         complex = float[2]
+ *   pfft_   double        complex = double[2] (double _Complex after <complex.h>)
+ *   pfftf_  float         complex = float[2]  (float _Complex after <complex.h>)
  * The long-double pfftl_ family has no GPU arithmetic and is not exported.
  */
 #ifndef PFFT_H
@@ -49,8 +49,19 @@ typedef enum {
 } pfft_r2r_kind;
 typedef pfft_r2r_kind pfftf_r2r_kind;
 
+/* Complex element types as FFTW 3 defines fftw_complex (reference
+ * api/pfft.h:515-516 takes them from fftw3.h): the C99 complex types when
+ * <complex.h> was included first, so that programs written for the
+ * reference (which scale with `in[l] /= N`) compile unchanged; otherwise
+ * two-element arrays.  Both have the same layout, so the library does not
+ * depend on the choice. */
+#if !defined(FFTW_NO_Complex) && defined(_Complex_I) && defined(complex) && defined(I)
+typedef double _Complex pfft_complex;
+typedef float _Complex pfftf_complex;
+#else
 typedef double pfft_complex[2];
 typedef float pfftf_complex[2];
+#endif
 
 /* Planner flags: same bit values as the reference (api/pfft.h:528-545). */
 #define PFFT_TRANSPOSED_NONE      (0U)

@@ -81,7 +81,7 @@ static int launch_generic_t(const pfk_pass *p, const void *src, void *dst) {
   if (ntiles <= 0) return 0;
   const T *s = (const T *)src;
   T *d = (T *)dst;
-  const int64_t chunk = (int64_t)1 << 30;
+  const int64_t chunk = pfk_max_grid(NT);
   for (int64_t base = 0; base < ntiles; base += chunk) {
     const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
     dim3 grid((unsigned)nblk), block(NT);

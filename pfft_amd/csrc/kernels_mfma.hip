@@ -613,7 +613,7 @@ static int launch_mfma_reg_s(const pfk_pass *p, int64_t ntiles, const void *src,
     hipLaunchKernelGGL((k_c2c_mfma_reg<NF, MODE, SIGN, LM>), dim3((unsigned)g), dim3(NF), 0,
                        g_stream, *p, (int64_t)0, ntiles, ntiles, s, d);
   } else {
-    const int64_t chunk = (int64_t)1 << 30;
+    const int64_t chunk = pfk_max_grid(NF);
     for (int64_t base = 0; base < ntiles; base += chunk) {
       const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
       hipLaunchKernelGGL((k_c2c_mfma_reg<NF, MODE, SIGN, LM>), dim3((unsigned)nblk), dim3(NF), 0,
@@ -645,7 +645,7 @@ static int launch_mfma_nf(const pfk_pass *p, const void *src, void *dst) {
   int64_t ntiles = (p->text + C - 1) / C;
   for (int i = 0; i < p->nb; ++i) ntiles *= p->bext[i];
   if (ntiles <= 0) return 0;
-  const int64_t chunk = (int64_t)1 << 30;
+  const int64_t chunk = pfk_max_grid(512);
   for (int64_t base = 0; base < ntiles; base += chunk) {
     const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
     if (p->sign < 0)

@@ -215,28 +215,47 @@ struct Tile16 {
   int nl;
 };
 
-// Tile of 16 lines for work item `item` (as the other pass kernels)
-__device__ __forceinline__ Tile16 tile16_at(const pfk_pass &P, int64_t item, int64_t ntiles) {
+// Tile `item` of CL consecutive lines along the tile dimension (the other
+// pass kernels' decode).  Indices below 2^32 (every practical pass) take
+// the 32-bit division chain, which is several times shorter than the
+// 64-bit one.
+__device__ __forceinline__ Tile16 tile_at_cl(const pfk_pass &P, int CL, int64_t item,
+                                             int64_t ntiles) {
   int64_t tileid = item;
   if (P.xcd_remap) {
     const int64_t qn = ntiles >> 3, rn = ntiles & 7, x = item & 7, jx = item >> 3;
     tileid = x * qn + (x < rn ? x : rn) + jx;
   }
-  const int64_t ntt = (P.text + 15) / 16;
-  const int64_t tt = tileid % ntt;
-  int64_t bflat = tileid / ntt;
-  const int64_t c0 = tt * 16;
+  const int64_t ntt = (P.text + CL - 1) / CL;
   Tile16 r;
-  r.nl = (int)((P.text - c0) < 16 ? (P.text - c0) : 16);
-  r.sb = c0 * P.tsin;
-  r.db = c0 * P.tsout;
-  for (int i = 0; i < P.nb; ++i) {
-    const int64_t e = P.bext[i];
-    const int64_t d = bflat % e;
-    bflat /= e;
-    r.sb += d * P.bsin[i];
-    r.db += d * P.bsout[i];
+  int64_t tt, sb = 0, db = 0;
+  if (ntiles <= (int64_t)0xFFFFFFFFu) {
+    uint32_t id = (uint32_t)tileid;
+    const uint32_t n32 = (uint32_t)ntt;
+    tt = id % n32;
+    id /= n32;
+    for (int i = 0; i < P.nb; ++i) {
+      const uint32_t e = (uint32_t)P.bext[i];
+      const uint32_t d = id % e;
+      id /= e;
+      sb += (int64_t)d * P.bsin[i];
+      db += (int64_t)d * P.bsout[i];
+    }
+  } else {
+    tt = tileid % ntt;
+    int64_t bflat = tileid / ntt;
+    for (int i = 0; i < P.nb; ++i) {
+      const int64_t e = P.bext[i];
+      const int64_t d = bflat % e;
+      bflat /= e;
+      sb += d * P.bsin[i];
+      db += d * P.bsout[i];
+    }
   }
+  const int64_t c0 = tt * CL;
+  r.nl = (int)((P.text - c0) < CL ? (P.text - c0) : CL);
+  r.sb = sb + c0 * P.tsin;
+  r.db = db + c0 * P.tsout;
   return r;
 }
 
@@ -256,7 +275,7 @@ k_mfma_tile(const pfk_pass P, const int64_t item0, const int64_t item_end, const
   cf *d0 = reinterpret_cast<cf *>(dst);
 
   int64_t item = item0 + blockIdx.x;
-  Tile16 tc = tile16_at(P, item, ntiles);
+  Tile16 tc = tile_at_cl(P, 16, item, ntiles);
   // loads of tile `t`: lane (q, c) of wave w, group g, slot s holds
   // x[c][4w + g + Q P(q, s)]
   auto load = [&](const Tile16 &t, int lt, cf *x) {
@@ -279,7 +298,7 @@ k_mfma_tile(const pfk_pass P, const int64_t item0, const int64_t item_end, const
     const bool q0 = q == 0;
     const int64_t ni = item + gridDim.x;
     const bool more = ni < item_end;
-    const Tile16 tn = tile16_at(P, more ? ni : item, ntiles);
+    const Tile16 tn = tile_at_cl(P, 16, more ? ni : item, ntiles);
     load(tn, lt, nx);
     // stage 1: DFT-16 over n2, twiddle W_NF^(n1 k2), to LDS
     // (the split form in two batches of two: its operand splits need the
@@ -385,33 +404,8 @@ struct LineGeo {
   static constexpr int R = NF / 256, LPW = 1024 / NF, CL = 4 * LPW;
 };
 
-__device__ __forceinline__ Tile16 linetile_at(const pfk_pass &P, int CL, int64_t item,
-                                              int64_t ntiles) {
-  int64_t tileid = item;
-  if (P.xcd_remap) {
-    const int64_t qn = ntiles >> 3, rn = ntiles & 7, x = item & 7, jx = item >> 3;
-    tileid = x * qn + (x < rn ? x : rn) + jx;
-  }
-  const int64_t ntt = (P.text + CL - 1) / CL;
-  const int64_t tt = tileid % ntt;
-  int64_t bflat = tileid / ntt;
-  const int64_t c0 = tt * CL;
-  Tile16 r;
-  r.nl = (int)((P.text - c0) < CL ? (P.text - c0) : CL);
-  r.sb = c0 * P.tsin;
-  r.db = c0 * P.tsout;
-  for (int i = 0; i < P.nb; ++i) {
-    const int64_t e = P.bext[i];
-    const int64_t d = bflat % e;
-    bflat /= e;
-    r.sb += d * P.bsin[i];
-    r.db += d * P.bsout[i];
-  }
-  return r;
-}
-
 template <int NF, int MODE, int SIGN>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 4)
 k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
             const float *__restrict__ src, float *__restrict__ dst) {
   constexpr int R = LineGeo<NF>::R, LPW = LineGeo<NF>::LPW, CL = LineGeo<NF>::CL;
@@ -425,7 +419,7 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
   const cf *s0 = reinterpret_cast<const cf *>(src);
   cf *d0 = reinterpret_cast<cf *>(dst);
   int64_t item = item0 + blockIdx.x;
-  Tile16 tc = linetile_at(P, CL, item, ntiles);
+  Tile16 tc = tile_at_cl(P, CL, item, ntiles);
   // lane (q, j16): first-stage butterfly rho = P(j16/4, j16%4), points
   // rho + 16 P(q, t) + 256 h of each of the wave's lines
   auto load = [&](const Tile16 &t, int lt, cf *x) {
@@ -451,7 +445,7 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
     const int rho = ptab(j16 >> 2, j16 & 3);
     const int64_t ni = item + gridDim.x;
     const bool more = ni < item_end;
-    const Tile16 tn = linetile_at(P, CL, more ? ni : item, ntiles);
+    const Tile16 tn = tile_at_cl(P, CL, more ? ni : item, ntiles);
     load(tn, lt, nx);
     const float un = MODE == 1 ? 16.f : 1.f;
     // radix-R step (DIF) of every line: sub-transform j gets
@@ -559,10 +553,6 @@ static int64_t resident_grid(K kernel, int threads) {
   return (int64_t)occ * (ncu / 8) * 8;
 }
 
-// largest grid (workgroups) of one launch: work items per dimension must
-// fit in 32 bits
-__host__ static int64_t max_blocks(int threads) { return (int64_t)(0xFFFFFFFFu / (unsigned)threads); }
-
 template <int NF, int MODE, int SIGN>
 static int launch_tile(const pfk_pass *p, const void *src, void *dst) {
   int64_t ntiles = (p->text + 15) / 16;
@@ -577,7 +567,7 @@ static int launch_tile(const pfk_pass *p, const void *src, void *dst) {
     hipLaunchKernelGGL((k_mfma_tile<NF, MODE, SIGN>), dim3((unsigned)g), dim3(NF), 0, g_stream,
                        *p, (int64_t)0, ntiles, ntiles, s, d);
   } else {
-    const int64_t chunk = max_blocks(NF);
+    const int64_t chunk = pfk_max_grid(NF);
     for (int64_t base = 0; base < ntiles; base += chunk) {
       const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
       hipLaunchKernelGGL((k_mfma_tile<NF, MODE, SIGN>), dim3((unsigned)nblk), dim3(NF), 0,
@@ -604,7 +594,7 @@ static int launch_line(const pfk_pass *p, const void *src, void *dst) {
     hipLaunchKernelGGL((k_mfma_line<NF, MODE, SIGN>), dim3((unsigned)g), dim3(256), 0, g_stream,
                        *p, (int64_t)0, ntiles, ntiles, s, d);
   } else {
-    const int64_t chunk = max_blocks(256);
+    const int64_t chunk = pfk_max_grid(256);
     for (int64_t base = 0; base < ntiles; base += chunk) {
       const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
       hipLaunchKernelGGL((k_mfma_line<NF, MODE, SIGN>), dim3((unsigned)nblk), dim3(256), 0,

@@ -268,26 +268,43 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n) {
   return x * q + (x < r ? x : r) + j;
 }
 
-// Tile of work item `item` (0 <= item < ntiles).
+// Tile of work item `item` (0 <= item < ntiles).  Tile counts below 2^32
+// (every practical pass) decode with 32-bit divisions, several times
+// shorter than the 64-bit chain.
 __device__ __forceinline__ TileCtx tile_at(const pfk_pass &P, int C, int64_t item,
                                            int64_t ntiles) {
   const int64_t ntt = (P.text + C - 1) / C;
   const int64_t tileid = P.xcd_remap ? xcd_remap(item, ntiles) : item;
-  const int64_t tt = tileid % ntt;
-  int64_t bflat = tileid / ntt;
+  int64_t tt, sb = 0, db = 0;
+  if (ntiles <= (int64_t)0xFFFFFFFFu) {
+    uint32_t id = (uint32_t)tileid;
+    const uint32_t n32 = (uint32_t)ntt;
+    tt = id % n32;
+    id /= n32;
+    for (int i = 0; i < P.nb; ++i) {
+      const uint32_t e = (uint32_t)P.bext[i];
+      const uint32_t d = id % e;
+      id /= e;
+      sb += (int64_t)d * P.bsin[i];
+      db += (int64_t)d * P.bsout[i];
+    }
+  } else {
+    tt = tileid % ntt;
+    int64_t bflat = tileid / ntt;
+    for (int i = 0; i < P.nb; ++i) {
+      const int64_t e = P.bext[i];
+      const int64_t d = bflat % e;
+      bflat /= e;
+      sb += d * P.bsin[i];
+      db += d * P.bsout[i];
+    }
+  }
   const int64_t c0 = tt * C;
   TileCtx r;
   r.c0 = c0;
   r.nl = (int)((P.text - c0) < C ? (P.text - c0) : C);
-  r.sb = c0 * P.tsin;
-  r.db = c0 * P.tsout;
-  for (int i = 0; i < P.nb; ++i) {
-    const int64_t e = P.bext[i];
-    const int64_t d = bflat % e;
-    bflat /= e;
-    r.sb += d * P.bsin[i];
-    r.db += d * P.bsout[i];
-  }
+  r.sb = sb + c0 * P.tsin;
+  r.db = db + c0 * P.tsout;
   return r;
 }
 
@@ -707,7 +724,7 @@ static int launch_items(void (*k)(pfk_pass, int64_t, int64_t, int64_t, const T *
     hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(threads), 0, g_stream, *p, (int64_t)0, ntiles,
                        ntiles, src, dst);
   } else {
-    const int64_t chunk = (int64_t)1 << 30;
+    const int64_t chunk = pfk_max_grid(threads);
     for (int64_t base = 0; base < ntiles; base += chunk) {
       const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
       hipLaunchKernelGGL(k, dim3((unsigned)nblk), dim3(threads), 0, g_stream, *p, base,

@@ -24,6 +24,12 @@ k_small_real(const pfk_pass P, const int64_t tile_base, const T *__restrict__ sr
   small_split_real_tile<T, NF, KIND>(P, tile_base + blockIdx.x, src, dst, smem_raw);
 }
 
+// The length rule of small_covers for even r2c / c2r passes (without the
+// device's LDS limit), for callers that enumerate the covered lengths.
+extern "C" int pfk_small_real_length(int nf, int f64) {
+  return split_real_length(nf, f64 != 0) && (small_length(nf, f64 != 0) || mid_length(nf, f64 != 0));
+}
+
 int small_covers(const pfk_pass *p) {
   if (p->n1 || p->dec_in || p->dec_out || p->twn) return 0;   // steps of long lines
   if (p->nf > 4096) return 0;
@@ -53,7 +59,7 @@ static int launch_small_nf(const pfk_pass *p, const void *src, void *dst) {
     const bool real = p->kind != PFK_C2C;   // real passes take the one-chunk shape
     const int lpt = small_lines_per_tile(NF, F64, real);
     const int64_t ntiles = (nlines + lpt - 1) / lpt;
-    const int64_t chunk = (int64_t)1 << 30;
+    const int64_t chunk = pfk_max_grid(small_threads(NF, F64, real));
     for (int64_t base = 0; base < ntiles; base += chunk) {
       const int64_t nblk = (ntiles - base) < chunk ? (ntiles - base) : chunk;
       const dim3 grid((unsigned)nblk), blk(small_threads(NF, F64, real));

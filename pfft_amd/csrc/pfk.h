@@ -281,6 +281,16 @@ int  pfk_event_destroy(void *ev);
 /* Complex conjugation / scaling utility kernels on device arrays. */
 int  pfk_scale(void *data, int prec, int64_t nreal, double factor);
 
+/* Largest grid (workgroups) of one launch of `threads`-thread workgroups:
+ * the work-item count per grid dimension must fit in 32 bits, so passes
+ * with more tiles are launched in chunks of this many workgroups. */
+int64_t pfk_max_grid(int threads);
+
+/* 1 if an even r2c / c2r pass of half length nf (f64 nonzero: complex
+ * double) has the short-length split kernel by length (the dispatcher's
+ * rule, before the device's LDS limit). */
+int pfk_small_real_length(int nf, int f64);
+
 /* Streaming bandwidth probe (GB/s of bytes read plus written), best of
  * `iters` launches: mode 0 read, 1 write, 2 copy; width 16/32/64 bytes per
  * lane and access; nt nontemporal; grid workgroups (0: one pass). */

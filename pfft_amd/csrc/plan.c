@@ -939,11 +939,13 @@ void pa_plans_execute_batch(const pa_plan *const *plans, int count) {
         !pfk_is_device_ptr(plans[i]->out))
       uniform = 0;
     for (int k = 0; k < plans[i]->npass; k++)
-      if (plans[i]->pass[k].n1 || plans[i]->pass[k].r2r_path)
-        uniform = 0;   /* passes that run as several launches */
+      if (plans[i]->pass[k].n1 || plans[i]->pass[k].r2r_path || plans[i]->pass[k].mfma)
+        uniform = 0;   /* passes that run as several launches, or in an MFMA
+                          arithmetic form the batch kernels do not have */
   }
   if (!uniform) {
-    /* host arrays, mixed precisions or long lines: one plan after another */
+    /* host arrays, mixed precisions, long lines or MFMA forms: one plan
+     * after another (each in its own arithmetic form) */
     for (int i = 0; i < count; i++) run(plans[i], plans[i]->in, plans[i]->out, 0);
     return;
   }

@@ -53,3 +53,8 @@ void pa_twiddles_step(long N, long count, long step, int sign, double *out) {
     out[2 * j + 1] = (double)(sign < 0 ? -ss : ss);
   }
 }
+
+int64_t pfk_max_grid(int threads) {
+  if (threads < 1) threads = 1;
+  return (int64_t)(0xFFFFFFFFu / (unsigned)threads);
+}

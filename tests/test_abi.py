@@ -116,3 +116,24 @@ int main(int argc, char **argv) {
     # permuted: sizes (1, 1, 2), starts (0, 0, 3)
     assert "c(0,0:0,3:4):" in r.stdout
     assert "1.00e+00 + -1.00e+00i," in r.stdout and "2.00e+00 + -2.00e+00i," in r.stdout
+
+
+def test_launch_chunks_fit_32bit_grids():
+    """Passes with more tiles than one launch can hold run in chunks of
+    pfk_max_grid(threads) workgroups: each chunk's work-item count fits in
+    32 bits and the chunks cover every tile exactly once."""
+    import ctypes
+    L = P.lib()
+    L.pfk_max_grid.restype = ctypes.c_int64
+    L.pfk_max_grid.argtypes = [ctypes.c_int]
+    for threads in (64, 128, 256, 320, 512, 768, 1024):
+        g = L.pfk_max_grid(threads)
+        assert g * threads <= 0xFFFFFFFF < (g + 1) * threads
+        for ntiles in (1, g - 1, g, g + 1, 3 * g + 7):
+            covered, base = 0, 0
+            while base < ntiles:
+                nblk = min(ntiles - base, g)
+                assert 0 < nblk * threads <= 0xFFFFFFFF
+                covered += nblk
+                base += g
+            assert covered == ntiles

@@ -53,6 +53,22 @@ def test_batch_matches_oracle_and_sequential():
         assert rel_l2(a, b) < 2e-6
 
 
+def oracle_f64(n, x):
+    """the oracle in FP64 (a 128^3 plan in long double would take minutes)"""
+    from oracle import semantics as S
+    return S.execute_flat(spec_of("c2c", n), x.astype(np.float64), precision=np.float64)
+
+
+def test_batch_40_plans_match_oracle():
+    """40 plans with sides up to 128 (every side of the uneven-batch
+    configuration) against the oracle, through the captured graph."""
+    items = make_batch(40, 2)
+    assert {s for it in items for s in it[0]} == set(SIDES)
+    P.execute_batch([it[4] for it in items])
+    for (n, x, _, _, _), g in zip(items, outputs(items)):
+        assert rel_l2(g, oracle_f64(n, x)) < 2e-6, n
+
+
 def test_batch_graph_replays_with_eager_work_between():
     items = make_batch(40, 2)
     plans = [it[4] for it in items]
@@ -60,14 +76,25 @@ def test_batch_graph_replays_with_eager_work_between():
         it[4].execute()
     eager = outputs(items)
     other = make_batch(3, 99, max_side=48)
-    for rep in range(3):
+    rng = np.random.default_rng(7)
+    for rep in range(4):
         P.execute_batch(plans)            # first call captures, later calls replay
         got = outputs(items)
         for a, b in zip(got, eager):
             assert rel_l2(a, b) < 1e-6, rep
-        # unrelated eager work and changed inputs between replays
+        # unrelated eager work between replays
         for it in other:
             it[4].execute()
+        # new inputs for a few plans: the next replay must read them
+        # (eager executes give the new reference)
+        for i in range(rep, 40, 9):
+            n, _, din, dout, plan = items[i]
+            x = rng.uniform(-1, 1, 2 * int(np.prod(n))).astype(np.float32)
+            din.upload(x)
+            plan.execute()
+            eager[i] = dout.download(np.float32, x.size).astype(np.float64)
+            items[i] = (n, x, din, dout, plan)
+        # and cleared outputs: the replay must rewrite them
         for it in items[:5]:
             it[3].upload(np.zeros(2 * int(np.prod(it[0])), dtype=np.float32))
 

@@ -1,11 +1,14 @@
 """GPU parity: the HIP path (through the pfft.h C ABI) against the oracle
 on the same seeded inputs.  FP64 bar: relative L2 <= 1e-12; FP32 bar:
 relative L2 <= 2e-6 (single-precision Stockham, inputs of O(1))."""
+import os
+
 import numpy as np
 import pytest
 
 import pfft_amd as P
 from oracle import layout as L
+from oracle import semantics as S
 from gpu_util import spec_of, random_input, run_gpu, oracle_out, out_mask, rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -129,9 +132,9 @@ def test_medium_r2c_c2r(n):
 
 def test_roundtrip_512_r2c_c2r():
     """Configuration 2 at full size: r2c (TRANSPOSED_OUT) then c2r
-    (TRANSPOSED_IN) returns N * input (size-independent property), and
-    one row of the spectrum matches a direct DFT of the corresponding row
-    sums."""
+    (TRANSPOSED_IN) returns N * input (size-independent property), two
+    lines of the spectrum match direct DFTs of plane sums, and the whole
+    forward spectrum matches the FP64 oracle to 1e-12 relative L2."""
     n = [512, 512, 512]
     N = 512 ** 3
     rc, comm = P.create_procmesh([1, 1])
@@ -166,6 +169,15 @@ def test_roundtrip_512_r2c_c2r():
     got2 = k[2 * off: 2 * off + 1024].reshape(512, 2)
     got2 = got2[:, 0] + 1j * got2[:, 1]
     assert np.linalg.norm(got2 - want2) / np.linalg.norm(want2) < 1e-12
+    # the whole forward spectrum against the oracle in FP64 (on this GPU's
+    # share of the host threads)
+    workers = S.WORKERS
+    S.WORKERS = min(16, len(os.sched_getaffinity(0)))
+    try:
+        want_k = S.execute_flat(spec_of("r2c", n, flags=TO), x, precision=np.float64)
+    finally:
+        S.WORKERS = workers
+    assert rel_l2(k, want_k) < 1e-12
     for o in (fw, bw):
         o.destroy()
     for b in (dx, dk, dy):
@@ -225,7 +237,7 @@ def test_long_pow2_real(n, prec):
 
 
 # ---- complex64 MFMA arithmetic forms (radix-16 stages on the matrix cores) ----
-@pytest.mark.parametrize("form,tol", [(P.AMD_MFMA_F32, 5e-6), (P.AMD_MFMA_BF16X2, 1e-4)])
+@pytest.mark.parametrize("form,tol", [(P.AMD_MFMA_F32, 5e-6), (P.AMD_MFMA_BF16X2, 1e-5)])
 @pytest.mark.parametrize("n,flags", [([512, 64, 256], 0), ([256, 512, 32], TO),
                                      ([32, 256, 1024], TI), ([16, 2048, 8], 0),
                                      ([4096, 4, 8], TO), ([40, 1024, 24], 0)])
@@ -237,6 +249,31 @@ def test_mfma_forms(n, flags, form, tol):
         want = oracle_out(spec, x)
         err = rel_l2(got, want)
         assert err <= tol, (n, flags, form, sign, err)
+
+
+@pytest.mark.parametrize("n", [[1024, 16, 512], [256, 32, 1024], [512, 8, 256]])
+def test_mfma_forms_precision_input(n):
+    """the precision configuration's input (complex64 U[-1,1)^2 plus a
+    1e-3-RMS 1/|k|^2 component) at reduced sizes whose passes run the
+    chained MFMA kernels (strided and contiguous 256/512/1024-point
+    lines): forward and backward against the oracle (FP64) at the 1e-5
+    relative-L2 bar, FP32-MFMA tighter."""
+    import torch
+    import bench_configs
+    x = bench_configs.cfg3_input(n, 11, torch.device("cuda:0")).cpu().numpy()
+    flat = np.empty(2 * x.size)
+    flat[0::2], flat[1::2] = x.real.ravel(), x.imag.ravel()
+    workers = S.WORKERS
+    S.WORKERS = min(16, len(os.sched_getaffinity(0)))
+    try:
+        for sign in (-1, +1):
+            spec = spec_of("c2c", n, sign=sign)
+            want = S.execute_flat(spec, flat, precision=np.float64)
+            for form, tol in ((P.AMD_MFMA_F32, 1e-6), (P.AMD_MFMA_BF16X2, 1e-5)):
+                got = run_gpu(spec, flat, prec="f32", plan_flags_extra=form)
+                assert rel_l2(got, want) <= tol, (n, sign, form, rel_l2(got, want))
+    finally:
+        S.WORKERS = workers
 
 
 def test_mfma_form_rejected_for_fp64():

@@ -17,8 +17,20 @@ pytestmark = pytest.mark.gpu
 
 TO, TI, SI, SO, EST = (P.TRANSPOSED_OUT, P.TRANSPOSED_IN, P.SHIFTED_IN, P.SHIFTED_OUT,
                        P.ESTIMATE)
-F32 = [18, 20, 24, 30, 32, 36, 48, 50, 60, 72, 90, 96, 100, 120, 144, 240, 480, 960]
-F64 = [18, 20, 24, 30, 32, 36, 48, 54, 60, 72, 96, 108, 144, 240, 480, 960]
+
+
+def covered_lengths(f64):
+    """Every half length the dispatcher sends to the split real kernel (the
+    library's own rule, pfk_small_real_length)."""
+    import ctypes
+    L = P.lib()
+    L.pfk_small_real_length.restype = ctypes.c_int
+    L.pfk_small_real_length.argtypes = [ctypes.c_int, ctypes.c_int]
+    return [nf for nf in range(1, 4097) if L.pfk_small_real_length(nf, int(f64))]
+
+
+F32 = covered_lengths(False)
+F64 = covered_lengths(True)
 
 
 def tol(prec):
@@ -58,14 +70,43 @@ def cases():
         yield nf, "f64"
 
 
+def test_covered_length_lists():
+    """the enumeration finds the lengths of the dispatcher's lists,
+    including the mid lengths and the whole-chunk shapes 64 and 128"""
+    for lst in (F32, F64):
+        for nf in (18, 24, 64, 128, 144, 160, 200, 216, 480, 960):
+            assert nf in lst, nf
+    assert 40 in F32 and 80 in F64 and 864 in F32
+
+
 @pytest.mark.parametrize("nf,prec", list(cases()))
 def test_real_split_lengths(nf, prec):
     n = [3, 5, 2 * nf]
     for kind, flags in (("r2c", 0), ("r2c", TO), ("c2r", 0), ("c2r", TI)):
         spec = spec_of(kind, n, sign=(+1 if kind == "c2r" else -1), flags=flags)
-        assert all(k == "small" for k in real_kernel(spec, prec)), (kind, flags)
+        names = real_kernel(spec, prec)
+        if nf & (nf - 1) == 0 and nf >= 64:
+            # plain power-of-two real lines run the power-of-two kernels;
+            # embedded ones (below) take the split kernel
+            assert names and all(k.startswith("fast") for k in names), (kind, flags, names)
+        else:
+            assert all(k == "small" for k in names), (kind, flags, names)
         err = check(kind, n, prec, flags)
         assert err <= tol(prec), (kind, flags, err)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("nf", [64, 128])
+def test_real_split_whole_chunk_shapes(nf, prec):
+    """64 and 128: the split kernel runs them when the power-of-two kernel
+    declines the pass (embedded real lines, or the adjoint's weights)."""
+    L = 2 * nf
+    for kind, ni, no in (("r2c", [3, 5, L - 6], [3, 5, L]), ("c2r", [3, 5, L], [3, 5, L - 10])):
+        n = [3, 5, L]
+        spec = spec_of(kind, n, ni=ni, no=no, sign=(+1 if kind == "c2r" else -1))
+        assert all(k == "small" for k in real_kernel(spec, prec)), (kind, real_kernel(spec, prec))
+        err = check(kind, n, prec, 0, ni=ni, no=no)
+        assert err <= tol(prec), (kind, err)
 
 
 @pytest.mark.parametrize("nf", [24, 60, 120, 240, 480])
