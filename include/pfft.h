@@ -414,10 +414,27 @@ PFFT_AMD_DECLARE_API(pfftf_, float, pfftf_complex, ptrdiff_t)
 void pfft_execute_adjoint(const pfft_plan plan, void *in, void *out);
 void pfftf_execute_adjoint(const pfftf_plan plan, void *in, void *out);
 
-/* Execute `count` plans as one batch (one persistent device launch per
- * pass kind where the plans allow it).  Blocking. */
+/* Execute `count` plans as one batch: every plan's pass 0, then every
+ * pass 1, and so on, each level as persistent device launches (one per
+ * kernel) over a work-list of the level's tiles, all captured into a
+ * hipGraph on the first call with these plans and arrays and replayed
+ * later.  Plans the batch kernels do not cover run one after another.
+ * Blocking.  Replaces pfft_execute for many small plans; the reference has
+ * no batch call (api/api-basic.c:1109 executes one plan). */
 void pfftf_execute_batch(const pfftf_plan *plans, int count);
 void pfft_execute_batch(const pfft_plan *plans, int count);
+
+/* The batch without any allocation, for callers that cannot allocate
+ * (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same kernels and result
+ * as pfft_execute_batch (launched directly, not through a graph), with the
+ * work-list in a fixed device array of the library (1.1 MiB, 2048 task
+ * records of 560 bytes, per GPU, kept until the process ends): at most 2048 passes in total (a 3-d plan has 3), plans of
+ * one precision (pfft_plan or pfftf_plan handles) on device arrays, and no
+ * long-line, r2r or MFMA-form passes.  Blocking; returns 0 on success and 1
+ * with a message otherwise.  Calls that overlap on one GPU share the array
+ * and race. */
+int pfft_amd_execute_batch_noalloc(const void *const *plans, int count);
 
 /* Copies between host and device memory (blocking), for callers that
  * fill arrays from the host. 0 on success. */

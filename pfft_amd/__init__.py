@@ -398,11 +398,22 @@ def exported_symbols():
 
 def execute_batch(plans):
     """pfft_execute_batch / pfftf_execute_batch over Plan objects of one
-    precision (one persistent launch, replayed from a captured graph)."""
+    precision (persistent launches per pass level, replayed from a captured
+    graph)."""
     if not plans:
         return
     arr = (VP * len(plans))(*[p.handle for p in plans])
     getattr(lib(), plans[0].P + "execute_batch")(arr, len(plans))
+
+
+def execute_batch_noalloc(plans):
+    """pfft_amd_execute_batch_noalloc: the batch with no allocation (fixed
+    device work-list); returns the C return code (0 on success)."""
+    L = lib()
+    L.pfft_amd_execute_batch_noalloc.restype = ctypes.c_int
+    L.pfft_amd_execute_batch_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    arr = (VP * max(len(plans), 1))(*[p.handle for p in plans])
+    return L.pfft_amd_execute_batch_noalloc(arr, len(plans))
 
 
 # ---------------------------------------------------------------- ghost cells

@@ -177,6 +177,10 @@ int pfft_amd_describe_plan(const void *plan, char *buf, size_t len) {
   if (!plan || !buf || !len) return 1;
   return pa_plan_describe((const pa_plan *)plan, buf, len);
 }
+int pfft_amd_execute_batch_noalloc(const void *const *plans, int count) {
+  return pa_plans_execute_batch_noalloc((const pa_plan *const *)plans, count);
+}
+
 double pfft_amd_bw_probe(int mode, int width, int nt, long long grid, size_t bytes, int iters) {
   return pfk_bw_probe(mode, width, nt, grid, bytes, iters);
 }

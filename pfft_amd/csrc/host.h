@@ -62,6 +62,7 @@ void pa_plan_destroy(pa_plan *p);
 void *pa_plan_in(const pa_plan *p);
 void *pa_plan_out(const pa_plan *p);
 void pa_plans_execute_batch(const pa_plan *const *plans, int count);
+int pa_plans_execute_batch_noalloc(const pa_plan *const *plans, int count);
 int pa_plan_time_passes(const pa_plan *p, void *in, void *out, int iters, double *ms,
                         double *bytes);
 int pa_plan_npasses(const pa_plan *p);

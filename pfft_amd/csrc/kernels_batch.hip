@@ -202,52 +202,57 @@ static int task_class(const pfk_task *t) {
   return 0;
 }
 
-static int cmp_class(const void *a, const void *b) {
-  const pfk_task *x = (const pfk_task *)a, *y = (const pfk_task *)b;
-  const int cx = task_class(x), cy = task_class(y);
-  if (cx != cy) return cx < cy ? -1 : 1;
-  return (x->pad < y->pad) ? -1 : (x->pad > y->pad);   // keep the given order otherwise
+// Stable counting sort of one level's tasks by kernel class (classes are
+// 0 or a short length, below 256), through `tmp` (room for n tasks).
+static int sort_level(pfk_task *a, int n, pfk_task *tmp) {
+  int start[257];
+  memset(start, 0, sizeof(start));
+  for (int i = 0; i < n; i++) {
+    const int c = task_class(&a[i]);
+    if (c < 0 || c > 255) { set_error_msg("pfk_batch: task class out of range"); return 1; }
+    start[c + 1]++;
+  }
+  for (int c = 0; c < 256; c++) start[c + 1] += start[c];
+  for (int i = 0; i < n; i++) tmp[start[task_class(&a[i])]++] = a[i];
+  memcpy(a, tmp, sizeof(pfk_task) * (size_t)n);
+  return 0;
 }
 
-/* tasks must be grouped by level; task.dep marks the previous pass of the
- * same plan, so a task whose dep lies in the current level starts a new
- * level.  The array is reordered (by kernel within each level). */
-extern "C" pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_graph) {
-  if (pfk_init() || ntasks <= 0) return nullptr;
+/* Groups of a batch (no allocation): tasks must be grouped by level;
+ * task.dep marks the previous pass of the same plan, so a task whose dep
+ * lies in the current level starts a new level.  The array is reordered
+ * (by kernel within each level, through `tmp`, room for ntasks) and its
+ * tiling fields filled. */
+static int batch_plan_levels(pfk_batch *b, pfk_task *tasks, int ntasks, pfk_task *tmp) {
   const int prec = tasks[0].pass.prec;
-  /* levels */
   int lev_start[PFK_MAXSTAGES + 1], nlev = 0;
   lev_start[0] = 0;
   for (int i = 0; i < ntasks; i++) {
     if (tasks[i].pass.prec != prec || tasks[i].dep >= i) {
       set_error_msg("pfk_batch: bad task list");
-      return nullptr;
+      return 1;
     }
     if (tasks[i].dep >= lev_start[nlev]) {
-      if (nlev + 1 >= PFK_MAXSTAGES) { set_error_msg("pfk_batch: too many levels"); return nullptr; }
+      if (nlev + 1 >= PFK_MAXSTAGES) { set_error_msg("pfk_batch: too many levels"); return 1; }
       lev_start[++nlev] = i;
     }
-    tasks[i].pad = i;
   }
   lev_start[++nlev] = ntasks;
-  pfk_batch *b = (pfk_batch *)calloc(1, sizeof(pfk_batch));
-  if (!b) return nullptr;
   b->ntasks = ntasks;
-  b->use_graph = use_graph;
+  b->ngroups = 0;
   int dev = 0, ncu = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   for (int l = 0; l < nlev; l++) {
     const int a = lev_start[l], z = lev_start[l + 1];
-    qsort(tasks + a, (size_t)(z - a), sizeof(pfk_task), cmp_class);
+    if (sort_level(tasks + a, z - a, tmp)) return 1;
     int i = a;
     while (i < z) {
       const int cls = task_class(&tasks[i]);
       int j = i;
       if (b->ngroups >= PFK_MAX_GROUPS) {
         set_error_msg("pfk_batch: too many task groups");
-        free(b);
-        return nullptr;
+        return 1;
       }
       batch_group *g = &b->groups[b->ngroups++];
       g->level = l;
@@ -267,7 +272,7 @@ extern "C" pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_grap
           nt = (lines + lpt - 1) / lpt;
           ld = small::lds_bytes(cls, prec == PFK_F64);
         } else {
-          if (generic_tiling(&t->pass, &C, &pitch, &ld)) { free(b); return nullptr; }
+          if (generic_tiling(&t->pass, &C, &pitch, &ld)) return 1;
           nt = (t->pass.text + C - 1) / C;
           for (int k = 0; k < t->pass.nb; k++) nt *= t->pass.bext[k];
         }
@@ -282,7 +287,7 @@ extern "C" pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_grap
       g->i1 = item;
       g->lds = lds;
       g->kernel = (prec == PFK_F64) ? batch_kernel<double>(cls) : batch_kernel<float>(cls);
-      if (!g->kernel) { set_error_msg("pfk_batch: no kernel"); free(b); return nullptr; }
+      if (!g->kernel) { set_error_msg("pfk_batch: no kernel"); return 1; }
       int occ = 0;
       hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, g->kernel, 256, lds);
       if (e != hipSuccess || occ < 1) occ = 1;
@@ -290,6 +295,18 @@ extern "C" pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_grap
       i = j;
     }
   }
+  return 0;
+}
+
+extern "C" pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_graph) {
+  if (pfk_init() || ntasks <= 0) return nullptr;
+  pfk_batch *b = (pfk_batch *)calloc(1, sizeof(pfk_batch));
+  if (!b) return nullptr;
+  b->use_graph = use_graph;
+  pfk_task *tmp = (pfk_task *)malloc(sizeof(pfk_task) * (size_t)ntasks);
+  const int bad = !tmp || batch_plan_levels(b, tasks, ntasks, tmp);
+  free(tmp);
+  if (bad) { free(b); return nullptr; }
   hipError_t e = hipMalloc(&b->d_tasks, sizeof(pfk_task) * ntasks);
   if (e == hipSuccess)
     e = hipMemcpy(b->d_tasks, tasks, sizeof(pfk_task) * ntasks, hipMemcpyHostToDevice);
@@ -305,6 +322,51 @@ extern "C" pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_grap
     if (e != hipSuccess) { set_error("pfk_batch graph capture", e); pfk_batch_destroy(b); return nullptr; }
   }
   return b;
+}
+
+// Batches of plans that cannot allocate: the device work-list lives in
+// this file-scope array instead of an allocation (one copy per GPU, kept
+// for the process; PFK_NOALLOC_TASKS passes at most).  Host code uses the
+// address hipGetSymbolAddress returns for the current device, never the
+// symbol itself.
+#define PFK_NOALLOC_TASKS 2048
+static __device__ __align__(256) unsigned char scratch[sizeof(pfk_task) * PFK_NOALLOC_TASKS];
+#define PFK_MAX_DEVICES 64
+static void *g_scratch_ptr[PFK_MAX_DEVICES];
+static pfk_batch g_noalloc_batch;                 // group table (host memory)
+static pfk_task g_noalloc_tmp[PFK_NOALLOC_TASKS];  // sort buffer (host memory)
+
+extern "C" int pfk_batch_run_noalloc(pfk_task *tasks, int ntasks) {
+  if (pfk_init()) return 1;
+  if (ntasks <= 0) return 0;
+  if (ntasks > PFK_NOALLOC_TASKS) {
+    set_error_msg("pfk_batch_run_noalloc: more passes than the fixed work-list holds (2048)");
+    return 1;
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) { set_error("pfk_batch_run_noalloc", e); return 1; }
+  if (dev < 0 || dev >= PFK_MAX_DEVICES) {
+    set_error_msg("pfk_batch_run_noalloc: device index out of range");
+    return 1;
+  }
+  if (!g_scratch_ptr[dev]) {   // once per device, with that device current
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) { set_error("pfk_batch_run_noalloc: scratch address", e); return 1; }
+    g_scratch_ptr[dev] = ptr;
+  }
+  pfk_batch *b = &g_noalloc_batch;
+  b->use_graph = 0;
+  if (batch_plan_levels(b, tasks, ntasks, g_noalloc_tmp)) return 1;
+  b->d_tasks = (pfk_task *)g_scratch_ptr[dev];
+  e = hipMemcpyAsync(b->d_tasks, tasks, sizeof(pfk_task) * (size_t)ntasks, hipMemcpyHostToDevice,
+                     g_stream);
+  if (e != hipSuccess) { set_error("pfk_batch_run_noalloc: work-list upload", e); return 1; }
+  if (batch_launch(b)) return 1;
+  e = hipStreamSynchronize(g_stream);
+  if (e != hipSuccess) { set_error("pfk_batch_run_noalloc", e); return 1; }
+  return 0;
 }
 
 extern "C" int pfk_batch_run(pfk_batch *b) {

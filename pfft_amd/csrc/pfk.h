@@ -182,13 +182,19 @@ typedef struct pfk_task {
 } pfk_task;
 
 /* Batched execution of many passes: tasks grouped by pass level, one
- * persistent launch per level walking a device work-list, all levels
- * optionally captured into one hipGraph and replayed.  All tasks must
- * share one precision. */
+ * persistent launch per level and kernel class walking a device work-list,
+ * all levels optionally captured into one hipGraph and replayed.  All tasks
+ * must share one precision. */
 typedef struct pfk_batch pfk_batch;
 pfk_batch *pfk_batch_create(pfk_task *tasks, int ntasks, int use_graph);
 int  pfk_batch_run(pfk_batch *b);
 void pfk_batch_destroy(pfk_batch *b);
+/* The same launches without any allocation: the device work-list goes
+ * into a fixed device array of the library (at most 2048 passes; tasks is
+ * host memory, in level order as for pfk_batch_create, and is reordered).
+ * Direct launches, no graph.  Blocking; calls that overlap on one GPU
+ * share the array and race. */
+int  pfk_batch_run_noalloc(pfk_task *tasks, int ntasks);
 
 /* Ghost cells of a 3-d local block on one process (sizes in the plan's
  * memory order, tuple reals innermost).  The block n[0] x n[1] x n[2] is

@@ -988,6 +988,70 @@ void pa_plans_execute_batch(const pa_plan *const *plans, int count) {
   if (pfk_batch_run(hit->b)) pa_error("batched execution failed: %s", pfk_last_error());
 }
 
+/* The batch for callers that cannot allocate: the same kernels and
+ * work-list as pa_plans_execute_batch, launched directly (no graph), with
+ * the task list built in a fixed host array and the device work-list in the
+ * library's fixed device array (pfk_batch_run_noalloc).  Returns 0 on success; 1 (with a
+ * message) when the plans need more than PA_NOALLOC_TASKS passes, hold host
+ * arrays, differ in precision or have passes the batch kernels do not run
+ * (long lines, r2r inner transforms, MFMA forms). */
+#define PA_NOALLOC_TASKS 2048
+static pfk_task g_noalloc_tasks[PA_NOALLOC_TASKS];
+
+int pa_plans_execute_batch_noalloc(const pa_plan *const *plans, int count) {
+  if (count <= 0) return 0;
+  if (!plans) { pa_error("execute_batch_noalloc: no plans"); return 1; }
+  int ntasks = 0, maxp = 0;
+  for (int i = 0; i < count; i++) {
+    const pa_plan *p = plans[i];
+    if (!p) { pa_error("execute_batch_noalloc: plan %d is NULL", i); return 1; }
+    if (p->prec != plans[0]->prec || !pfk_is_device_ptr(p->in) || !pfk_is_device_ptr(p->out)) {
+      pa_error("execute_batch_noalloc: plans must share one precision and use device arrays");
+      return 1;
+    }
+    for (int k = 0; k < p->npass; k++)
+      if (p->pass[k].n1 || p->pass[k].r2r_path || p->pass[k].mfma) {
+        pa_error("execute_batch_noalloc: plan %d has passes the batch kernels do not run", i);
+        return 1;
+      }
+    ntasks += p->npass;
+    if (p->npass > maxp) maxp = p->npass;
+  }
+  if (ntasks > PA_NOALLOC_TASKS) {
+    pa_error("execute_batch_noalloc: %d passes exceed the fixed work-list of %d", ntasks,
+             PA_NOALLOC_TASKS);
+    return 1;
+  }
+  /* level order (every plan's pass 0, then every pass 1, ...), each task
+   * depending on the previous pass of its plan: within the block of level
+   * lev-1 the plans with at least lev passes appear in plan order */
+  int k = 0, prev_start = 0;
+  for (int lev = 0; lev < maxp; lev++) {
+    const int cur_start = k;
+    int a = prev_start;
+    for (int i = 0; i < count; i++) {
+      const pa_plan *p = plans[i];
+      const int in_prev = lev > 0 && p->npass > lev - 1;
+      if (lev < p->npass) {
+        void *bufs[4] = {p->in, p->out, p->scratch[0], p->scratch[1]};
+        pfk_task *t = &g_noalloc_tasks[k++];
+        memset(t, 0, sizeof(*t));
+        t->pass = p->pass[lev];
+        t->src = bufs[p->src_role[lev]];
+        t->dst = bufs[p->dst_role[lev]];
+        t->dep = lev > 0 ? a : -1;
+      }
+      if (in_prev) a++;
+    }
+    prev_start = cur_start;
+  }
+  if (pfk_batch_run_noalloc(g_noalloc_tasks, ntasks)) {
+    pa_error("execute_batch_noalloc failed: %s", pfk_last_error());
+    return 1;
+  }
+  return 0;
+}
+
 /* Times every pass of the plan over `iters` back-to-back executions on
  * device arrays (HIP events on the library stream between passes).
  * ms[k] receives the mean duration of pass k; bytes[k] the bytes pass k
