@@ -178,7 +178,9 @@ def cfg3_forms(seed=20261202, steps=5, n=(1024, 1024, 512), reference=True, core
 
 
 def cfg3(args):
-    print(json.dumps(cfg3_forms(args.seed)))
+    """--steps fwd+bwd pairs per form; --no-cpu skips the host FP64
+    reference (for profiler runs)"""
+    print(json.dumps(cfg3_forms(args.seed, steps=args.steps, reference=not args.no_cpu)))
 
 
 def cfg4(args):

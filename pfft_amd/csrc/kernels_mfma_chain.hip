@@ -37,14 +37,14 @@
 // (16 lines x 8 bytes per row of the wave).
 //
 // Line-major form (contiguous lines, k_mfma_line).  A wave transforms
-// lines of NF = R 256 points without LDS: a radix-R step (DIF) inside the
-// lanes, then for each of the R sub-transforms of 256 points two chained
-// radix-16 products.  The data is the A operand (rows = butterflies): the
-// product's output rows 4q..4q+3 of lane (q, k2) are the butterflies
-// P(q, 0..3) of the first stage (rows are loaded in that order), so they
-// are directly the four slot points of the second stage.  X[R (k2 + 16 k1)
-// + j] sits in lane (q, k1), row k2 = 4q + v: 4R consecutive outputs per
-// lane, which a per-wave LDS row turns into stores of 64 consecutive
+// lines of NF = R 256 points without an LDS exchange: for each of the R
+// decimated sub-transforms of 256 points two chained radix-16 products,
+// then a radix-R step (DIT) inside the lanes.  The data is the A operand
+// (rows = butterflies): the product's output rows 4q..4q+3 of lane (q, k2)
+// are the butterflies P(q, 0..3) of the first stage (rows are loaded in
+// that order), so they are directly the four slot points of the second
+// stage.  Y_j[k2 + 16 k1] sits in lane (q, k1), row k2 = 4q + v; a
+// per-wave LDS row turns the outputs into stores of 64 consecutive
 // elements per instruction.
 //
 // Per-stage scaling of the split form: the DFT coefficients are multiplied
@@ -58,6 +58,7 @@ namespace {
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
 
 struct cf {
   float x, y;
@@ -91,11 +92,15 @@ __constant__ float kcos16[16] = {1.f, 0.92387953251128675613f, 0.707106781186547
                                  -0.38268343236508977173f, 0.f, 0.38268343236508977173f,
                                  0.70710678118654752440f, 0.92387953251128675613f};
 
-__device__ __forceinline__ uint16_t bf16_rne(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+// hi + lo split of four floats into BF16 operands: hi = bf16(x), lo =
+// bf16(x - hi), both rounded to nearest even (the conversion is the
+// packed v_cvt_pk_bf16_f32 of gfx950: 3 instructions per float)
+__device__ __forceinline__ void split_bf16(v4f x, v4s &hi, v4s &lo) {
+  const v4bf h = __builtin_convertvector(x, v4bf);
+  const v4f back = __builtin_convertvector(h, v4f);
+  hi = __builtin_bit_cast(v4s, h);
+  lo = __builtin_bit_cast(v4s, __builtin_convertvector(x - back, v4bf));
 }
-__device__ __forceinline__ float bf16_val(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
 // DFT-16 coefficients of this lane (row group q, output frequency k):
 // FP32 form cr/ci[s]; split-BF16 form hi/lo vectors (scaled by 1/4)
@@ -121,15 +126,8 @@ struct Coef {
 #pragma unroll
       for (int s = 0; s < 4; ++s) { cr[s] = r[s]; ci[s] = i[s]; }
     } else {
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const float a = 0.25f * r[s], b = 0.25f * i[s];
-        const uint16_t ah = bf16_rne(a), bh = bf16_rne(b);
-        rh[s] = (short)ah;
-        rl[s] = (short)bf16_rne(a - bf16_val(ah));
-        ih[s] = (short)bh;
-        il[s] = (short)bf16_rne(b - bf16_val(bh));
-      }
+      split_bf16(v4f{0.25f * r[0], 0.25f * r[1], 0.25f * r[2], 0.25f * r[3]}, rh, rl);
+      split_bf16(v4f{0.25f * i[0], 0.25f * i[1], 0.25f * i[2], 0.25f * i[3]}, ih, il);
     }
   }
 };
@@ -176,15 +174,10 @@ __device__ __forceinline__ void dft16_mfma(const Coef<MODE, SIGN> &K, const cf *
   } else {
     v4s rh[N], rl[N], ih[N], il[N];
 #pragma unroll
-    for (int n = 0; n < N; ++n)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const uint16_t a = bf16_rne(yr[n][s]), b = bf16_rne(yi[n][s]);
-        rh[n][s] = (short)a;
-        rl[n][s] = (short)bf16_rne(yr[n][s] - bf16_val(a));
-        ih[n][s] = (short)b;
-        il[n][s] = (short)bf16_rne(yi[n][s] - bf16_val(b));
-      }
+    for (int n = 0; n < N; ++n) {
+      split_bf16(v4f{yr[n][0], yr[n][1], yr[n][2], yr[n][3]}, rh[n], rl[n]);
+      split_bf16(v4f{yi[n][0], yi[n][1], yi[n][2], yi[n][3]}, ih[n], il[n]);
+    }
 #define PFK_BF(x, y, acc) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, y, acc, 0, 0, 0)
 #pragma unroll
     for (int n = 0; n < N; ++n) {
@@ -395,20 +388,45 @@ k_mfma_tile(const pfk_pass P, const int64_t item0, const int64_t item_end, const
 // ---------------------------------------------------------------- line-major
 // Lines with sin = sout = 1.  A workgroup (4 waves) takes tiles of
 // CL = 4 * LPW consecutive lines along the tile dimension; wave w
-// transforms lines w*LPW .. w*LPW+LPW-1 of the tile (LPW = 1024/NF, so a
-// lane always holds 16 complex values), and the loads of the workgroup's
-// next tile are issued before the current one is transformed.  No LDS
-// besides the twiddle table, no barrier in the loop.
+// transforms lines w*LPW .. w*LPW+LPW-1 of the tile, NP = LPW * R radix-16
+// products per stage (NP = 2 for 256 and 512 points, 4 for 1024: a lane
+// holds 4 NP complex values), and the loads of the workgroup's next tile
+// are issued before the current one is transformed (two register buffers
+// of 4 NP values; with NP = 4 at 256 or 512 points the pair no longer fits
+// the register budget of four waves per SIMD and spills).  No LDS
+// exchange besides the twiddle table and the per-wave store row, no
+// workgroup barrier in the loop.
+//
+// Decimation in time: sub-transform j (j < R) is the 256-point DFT of
+// x[R n' + j], so the R values a lane needs at one n' are adjacent in
+// memory (one 16-byte load for R = 2, two for R = 4), and the radix-R
+// step runs after the products: X[k + 256 m] = sum_j W_NF^(jk) Y_j[k]
+// W_R^(jm), with Y_j[k] at the same register position of every j.
 template <int NF>
 struct LineGeo {
-  static constexpr int R = NF / 256, LPW = 1024 / NF, CL = 4 * LPW;
+  static constexpr int R = NF / 256, NP = NF == 1024 ? 4 : 2, LPW = NP / R, CL = 4 * LPW;
 };
+
+template <int R>
+__device__ __forceinline__ void load_run(const cf *p, cf *x) {   // x[4 j] = p[j], j < R
+  if constexpr (R == 1) {
+    x[0] = ldnt(p);
+  } else {
+#pragma unroll
+    for (int h = 0; h < R / 2; ++h) {
+      const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p) + h);
+      x[4 * (2 * h)] = cf{v.x, v.y};
+      x[4 * (2 * h + 1)] = cf{v.z, v.w};
+    }
+  }
+}
 
 template <int NF, int MODE, int SIGN>
 __global__ void __launch_bounds__(256, 4)
 k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
             const float *__restrict__ src, float *__restrict__ dst) {
-  constexpr int R = LineGeo<NF>::R, LPW = LineGeo<NF>::LPW, CL = LineGeo<NF>::CL;
+  constexpr int R = LineGeo<NF>::R, NP = LineGeo<NF>::NP, LPW = LineGeo<NF>::LPW;
+  constexpr int CL = LineGeo<NF>::CL;
   constexpr int SLOTS = NF + NF / 16;   // one line, one pad slot per 16 positions
   __shared__ cf twl[NF];
   __shared__ cf lx[4][SLOTS];           // per-wave transpose of a finished line
@@ -420,8 +438,9 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
   cf *d0 = reinterpret_cast<cf *>(dst);
   int64_t item = item0 + blockIdx.x;
   Tile16 tc = tile_at_cl(P, CL, item, ntiles);
-  // lane (q, j16): first-stage butterfly rho = P(j16/4, j16%4), points
-  // rho + 16 P(q, t) + 256 h of each of the wave's lines
+  // lane (q, j16): first-stage butterfly rho = P(j16/4, j16%4); slot u of
+  // sub-transform j of line i is x[R (rho + 16 P(q, u)) + j], held in
+  // x[(i R + j) 4 + u]
   auto load = [&](const Tile16 &t, int lt, cf *x) {
     const int q = (lt & 63) >> 4, j16 = lt & 15;
     const int w = __builtin_amdgcn_readfirstlane(lt >> 6);
@@ -429,11 +448,9 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
 #pragma unroll
     for (int i = 0; i < LPW; ++i) {
       const int ln = w * LPW + i;
-      const cf *s = s0 + t.sb + (int64_t)(ln < t.nl ? ln : 0) * P.tsin + rho;
+      const cf *s = s0 + t.sb + (int64_t)(ln < t.nl ? ln : 0) * P.tsin + R * rho;
 #pragma unroll
-      for (int h = 0; h < R; ++h)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) x[(i * R + h) * 4 + u] = ldnt(s + 256 * h + 16 * ptab(q, u));
+      for (int u = 0; u < 4; ++u) load_run<R>(s + 16 * R * ptab(q, u), x + i * R * 4 + u);
     }
   };
   auto step = [&](cf *x, cf *nx) -> bool {
@@ -442,72 +459,71 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
     const int q = (lt & 63) >> 4, j16 = lt & 15;
     const int w = __builtin_amdgcn_readfirstlane(lt >> 6);
     const bool q0 = q == 0;
-    const int rho = ptab(j16 >> 2, j16 & 3);
     const int64_t ni = item + gridDim.x;
     const bool more = ni < item_end;
     const Tile16 tn = tile_at_cl(P, CL, more ? ni : item, ntiles);
     load(tn, lt, nx);
+    // keep the prefetch ahead of the transform (the scheduler would
+    // otherwise sink the loads below it to save registers)
+    __builtin_amdgcn_sched_barrier(0);
     const float un = MODE == 1 ? 16.f : 1.f;
-    // radix-R step (DIF) of every line: sub-transform j gets
-    // sum_h x_h W_R^(hj) W_NF^(n' j)
+    // the NP = LPW * R sub-transforms of 256 points together: first
+    // products, twiddle W_256^(n1 k2) (rows 4q + v are butterflies
+    // P(q, v), column k2 = j16), second products
+    {
+      v4f ar[NP], ai[NP];
+      dft16_mfma<NP, MODE, SIGN, true>(K, x, q0, ar, ai);
 #pragma unroll
-    for (int i = 0; i < LPW; ++i) {
-      cf *xl = x + i * R * 4;
-      if constexpr (R > 1) {
+      for (int n = 0; n < NP; ++n)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int np = rho + 16 * ptab(q, u);
+        for (int v = 0; v < 4; ++v)
+          x[4 * n + v] = cmf(cf{ar[n][v], ai[n][v]}, twl[R * ptab(q, v) * j16]);
+      dft16_mfma<NP, MODE, SIGN, true>(K, x, q0, ar, ai);
+#pragma unroll
+      for (int n = 0; n < NP; ++n)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) x[4 * n + v] = cf{ar[n][v] * un, ai[n][v] * un};
+    }
+    // Y_j[k] (k = 4q + v + 16 j16) is x[(i R + j) 4 + v]; the radix-R step
+    // leaves X[k + 256 m] in x[(i R + m) 4 + v]
+    if constexpr (R > 1) {
+#pragma unroll
+      for (int i = 0; i < LPW; ++i) {
+        cf *xl = x + i * R * 4;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int k = 4 * q + v + 16 * j16;
           if constexpr (R == 2) {
-            const cf a = xl[u], b = xl[4 + u];
-            xl[u] = cf{a.x + b.x, a.y + b.y};
-            xl[4 + u] = cmf(cf{a.x - b.x, a.y - b.y}, twl[np]);
+            const cf a = xl[v], b = cmf(xl[4 + v], twl[k]);
+            xl[v] = cf{a.x + b.x, a.y + b.y};
+            xl[4 + v] = cf{a.x - b.x, a.y - b.y};
           } else {
-            const cf a0 = xl[u], a1 = xl[4 + u], a2 = xl[8 + u], a3 = xl[12 + u];
+            const cf a0 = xl[v], a1 = cmf(xl[4 + v], twl[k]);
+            const cf a2 = cmf(xl[8 + v], twl[2 * k]), a3 = cmf(xl[12 + v], twl[3 * k]);
             const cf t0{a0.x + a2.x, a0.y + a2.y}, t1{a0.x - a2.x, a0.y - a2.y};
             const cf t2{a1.x + a3.x, a1.y + a3.y};
             const cf t3 = mul_si<SIGN>(cf{a1.x - a3.x, a1.y - a3.y});
-            xl[u] = cf{t0.x + t2.x, t0.y + t2.y};
-            xl[4 + u] = cmf(cf{t1.x + t3.x, t1.y + t3.y}, twl[np]);
-            xl[8 + u] = cmf(cf{t0.x - t2.x, t0.y - t2.y}, twl[2 * np]);
-            xl[12 + u] = cmf(cf{t1.x - t3.x, t1.y - t3.y}, twl[3 * np]);
+            xl[v] = cf{t0.x + t2.x, t0.y + t2.y};
+            xl[8 + v] = cf{t0.x - t2.x, t0.y - t2.y};
+            xl[4 + v] = cf{t1.x + t3.x, t1.y + t3.y};
+            xl[12 + v] = cf{t1.x - t3.x, t1.y - t3.y};
           }
         }
       }
     }
-    // the LPW * R = 4 sub-transforms of 256 points together: first
-    // products, twiddle W_256^(n1 k2) (rows 4q + v are butterflies
-    // P(q, v), column k2 = j16), second products
-    {
-      v4f ar[4], ai[4];
-      dft16_mfma<4, MODE, SIGN, true>(K, x, q0, ar, ai);
-#pragma unroll
-      for (int n = 0; n < 4; ++n)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          x[4 * n + v] = cmf(cf{ar[n][v], ai[n][v]}, twl[R * ptab(q, v) * j16]);
-      dft16_mfma<4, MODE, SIGN, true>(K, x, q0, ar, ai);
-#pragma unroll
-      for (int n = 0; n < 4; ++n)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) x[4 * n + v] = cf{ar[n][v] * un, ai[n][v] * un};
-    }
-    // X[R (k2 + 16 k1) + j] (k2 = 4q + v, k1 = j16) sits at position
-    // R v + j of the lane's run of 4R; each line goes through the wave's
-    // LDS row (slot p + p/16: the 16 lanes of a row group write 16 runs
-    // of 4R values 17R slots apart, on distinct banks) and is stored as
-    // 64 consecutive elements per instruction.
+    // each line goes through the wave's LDS row (slot p + p/16) and is
+    // stored as 64 consecutive elements per instruction
     cf *row = lx[w];
     const int lane = lt & 63;
 #pragma unroll
     for (int i = 0; i < LPW; ++i) {
       const cf *xl = x + i * R * 4;
-      const int base = R * 4 * q + 16 * R * j16;
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
+      for (int m = 0; m < R; ++m)
 #pragma unroll
-        for (int jj = 0; jj < R; ++jj) {
-          const int pp = base + R * v + jj;
-          row[pp + (pp >> 4)] = xl[4 * jj + v];
+        for (int v = 0; v < 4; ++v) {
+          const int pp = 4 * q + v + 16 * j16 + 256 * m;
+          row[pp + (pp >> 4)] = xl[4 * m + v];
         }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -534,7 +550,7 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
     tc = tn;
     return true;
   };
-  cf xa[16], xb[16];
+  cf xa[4 * NP], xb[4 * NP];
   load(tc, tid, xa);
   lds_barrier();   // twiddle table
   while (step(xa, xb) && step(xb, xa)) {
@@ -626,13 +642,22 @@ int mfma_chain_covers(const pfk_pass *p) {
   return !p->imod.on && !p->omod.on;
 }
 
-// line-major form: lines contiguous on both sides
-static bool line_form_ok(const pfk_pass *p) { return p->sin == 1 && p->sout == 1; }
+// line-major form: lines contiguous on both sides; for 512 and 1024
+// points the loads take 16-byte vectors, so every line must start on a
+// 16-byte boundary (even element offsets)
+static bool line_form_ok(const pfk_pass *p, const void *src) {
+  if (p->sin != 1 || p->sout != 1) return false;
+  if (p->nf == 256) return true;
+  if (((uintptr_t)src & 15) || (p->tsin & 1)) return false;
+  for (int i = 0; i < p->nb; ++i)
+    if (p->bsin[i] & 1) return false;
+  return true;
+}
 
 // 0 launched, 1 not covered, -1 error
 int launch_mfma_chain(const pfk_pass *p, const void *src, void *dst) {
   if (!mfma_chain_covers(p)) return 1;
-  const bool line = (p->variant & 8) && line_form_ok(p);
+  const bool line = (p->variant & 8) && line_form_ok(p, src);
   const int mode = p->mfma - 1;
   switch (p->nf) {
     case 256:
