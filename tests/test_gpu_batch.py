@@ -116,3 +116,62 @@ def test_batch_without_graph_and_fp64():
     for (n, x, _, _, _), g in zip(items, got):
         want = oracle_out(spec_of("c2c", n, flags=P.TRANSPOSED_OUT), x)
         assert rel_l2(g, want) < 1e-12
+
+
+def test_uneven_384_batch_workload():
+    """The uneven-batch benchmark workload itself: 384 c2c complex64 plans
+    with every side drawn from {24, ..., 128} (the bench's generator and
+    seed), through the captured graph: the replay reproduces the first call
+    bit for bit, every plan agrees with its own eager execution, and a
+    sample of 24 plans (every largest side represented) matches the
+    oracle."""
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(20261202)
+    rc, comm = P.create_procmesh([1, 1])
+    plans, bufs, shapes = [], [], []
+    for _ in range(384):
+        n = [int(rng.choice(SIDES)) for _ in range(3)]
+        a = (torch.rand(n, device=dev, dtype=torch.complex64) * 2 - (1 + 1j)).to(torch.complex64)
+        b = torch.empty_like(a)
+        p = P.Plan("c2c", n, a.data_ptr(), b.data_ptr(), comm=comm, prec="f32")
+        assert p.ok
+        plans.append(p)
+        bufs.append((a, b))
+        shapes.append(n)
+    print("uneven 384-plan batch: %d points" % sum(int(np.prod(n)) for n in shapes), flush=True)
+    runs = []
+    for rep in range(2):      # the first call captures, the second replays
+        for _, b in bufs:
+            b.zero_()
+        P.execute_batch(plans)
+        torch.cuda.synchronize()
+        runs.append([b.clone() for _, b in bufs])
+    for i in range(384):
+        assert torch.equal(runs[0][i], runs[1][i]), (i, shapes[i])
+    # eager execution may pick other kernels (power-of-two lengths), so
+    # the batch agrees with it to FP32 accuracy rather than bit for bit
+    for p in plans:
+        p.execute()
+    torch.cuda.synchronize()
+    for i, (_, b) in enumerate(bufs):
+        d = (runs[0][i] - b).abs().pow(2).sum().sqrt() / b.abs().pow(2).sum().sqrt()
+        assert float(d) < 2e-6, (i, shapes[i], float(d))
+    # sample: the first plan of each side class plus a seeded draw
+    pick, seen = [], set()
+    for i, n in enumerate(shapes):
+        if max(n) not in seen:
+            seen.add(max(n))
+            pick.append(i)
+    pick += [int(i) for i in np.random.default_rng(5).choice(384, 24 - len(pick), replace=False)
+             if int(i) not in pick]
+    for i in pick:
+        x = bufs[i][0].cpu().numpy().astype(np.complex128).reshape(-1)
+        flat = np.empty(2 * x.size)
+        flat[0::2], flat[1::2] = x.real, x.imag
+        y = runs[0][i].cpu().numpy().astype(np.complex128).reshape(-1)
+        g = np.empty(2 * y.size)
+        g[0::2], g[1::2] = y.real, y.imag
+        assert rel_l2(g, oracle_f64(shapes[i], flat)) < 2e-6, shapes[i]
+    for p in plans:
+        p.destroy()
