@@ -512,7 +512,7 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
       }
     }
     // each line goes through the wave's LDS row (slot p + p/16) and is
-    // stored as 64 consecutive elements per instruction
+    // stored as 128 consecutive elements per instruction (two per lane)
     cf *row = lx[w];
     const int lane = lt & 63;
 #pragma unroll
@@ -529,16 +529,18 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const int ln = w * LPW + i;
-      cf *d = d0 + tc.db + (int64_t)(ln < tc.nl ? ln : 0) * P.tsout + lane;
-      cf o[NF / 64];
+      cf *d = d0 + tc.db + (int64_t)(ln < tc.nl ? ln : 0) * P.tsout + 2 * lane;
+      v4f o[NF / 128];   // positions 2 lane + 128 m and the one after
 #pragma unroll
-      for (int m = 0; m < NF / 64; ++m) {
-        const int pp = lane + 64 * m;
-        o[m] = row[pp + (pp >> 4)];
+      for (int m = 0; m < NF / 128; ++m) {
+        const int pp = 2 * lane + 128 * m;
+        const cf a = row[pp + (pp >> 4)], b = row[pp + 1 + ((pp + 1) >> 4)];
+        o[m] = v4f{a.x, a.y, b.x, b.y};
       }
       if (ln < tc.nl) {
 #pragma unroll
-        for (int m = 0; m < NF / 64; ++m) stnt(d + 64 * m, o[m]);
+        for (int m = 0; m < NF / 128; ++m)
+          __builtin_nontemporal_store(o[m], reinterpret_cast<v4f *>(d + 128 * m));
       }
       // the row is rewritten by the next line only after every lane read it
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -642,11 +644,14 @@ int mfma_chain_covers(const pfk_pass *p) {
   return !p->imod.on && !p->omod.on;
 }
 
-// line-major form: lines contiguous on both sides; for 512 and 1024
-// points the loads take 16-byte vectors, so every line must start on a
-// 16-byte boundary (even element offsets)
-static bool line_form_ok(const pfk_pass *p, const void *src) {
+// line-major form: lines contiguous on both sides; the stores (and for
+// 512 and 1024 points the loads) take 16-byte vectors, so every line must
+// start on a 16-byte boundary (even element offsets)
+static bool line_form_ok(const pfk_pass *p, const void *src, const void *dst) {
   if (p->sin != 1 || p->sout != 1) return false;
+  if (((uintptr_t)dst & 15) || (p->tsout & 1)) return false;
+  for (int i = 0; i < p->nb; ++i)
+    if (p->bsout[i] & 1) return false;
   if (p->nf == 256) return true;
   if (((uintptr_t)src & 15) || (p->tsin & 1)) return false;
   for (int i = 0; i < p->nb; ++i)
@@ -657,7 +662,7 @@ static bool line_form_ok(const pfk_pass *p, const void *src) {
 // 0 launched, 1 not covered, -1 error
 int launch_mfma_chain(const pfk_pass *p, const void *src, void *dst) {
   if (!mfma_chain_covers(p)) return 1;
-  const bool line = (p->variant & 8) && line_form_ok(p, src);
+  const bool line = (p->variant & 8) && line_form_ok(p, src, dst);
   const int mode = p->mfma - 1;
   switch (p->nf) {
     case 256:
