@@ -136,6 +136,8 @@ def cpu_baseline(n, cores, cpu_desc, seed):
     k = S.execute_flat(fw, x, precision=np.float64)
     y = S.execute_flat(bw, k, precision=np.float64)
     t1 = time.perf_counter()
+    global _ORACLE_FORWARD
+    _ORACLE_FORWARD = k
     err = float(np.max(np.abs(y / N - x)))
     xr = x.reshape(n, n, n)
     t2 = time.perf_counter()
@@ -159,6 +161,86 @@ def cpu_baseline(n, cores, cpu_desc, seed):
                                 "layout), %d workers; %.2f s; roundtrip max err %.1e; second "
                                 "labelled stand-in" % (n, cores, t3 - t2, err2)},
     }
+
+
+_ORACLE_FORWARD = None   # the oracle's forward output of the CPU leg (flat scalars)
+
+
+def pair_forms(L, n, x, steps, warmup, ms64, k64):
+    """The headline pair (n^3 r2c TRANSPOSED_OUT + c2r TRANSPOSED_IN) in the
+    complex64 forms: plain FP32, FP32 with MFMA radix-16 stages, split-BF16
+    MFMA, on the same input rounded to FP32.  Returns per form the ms per
+    pair over `steps` pairs, the speed-up over the FP64 pair (`ms64`), the
+    per-pass times and kernels, the forward output (host, flat) and the
+    roundtrip error.  k64: the FP64 path's forward output (flat)."""
+    import pfft_amd as P
+    N = n ** 3
+    rc, comm = P.create_procmesh([1, 1])
+    mem = P.local_size("r2c", [n] * 3, flags=P.TRANSPOSED_OUT, comm=comm)[0]
+    x32 = x.astype(np.float32)
+    forms = []
+    for name, extra in (("fp32 plain", 0), ("fp32 mfma radix-16", P.AMD_MFMA_F32),
+                        ("split-bf16 mfma", P.AMD_MFMA_BF16X2)):
+        dx = P.DeviceBuffer(N * 4)
+        dk = P.DeviceBuffer(mem * 8)
+        dy = P.DeviceBuffer(2 * mem * 4)
+        fw = P.Plan("r2c", [n] * 3, dx.ptr, dk.ptr, flags=P.TRANSPOSED_OUT | extra, comm=comm,
+                    prec="f32")
+        bw = P.Plan("c2r", [n] * 3, dk.ptr, dy.ptr, sign=+1, flags=P.TRANSPOSED_IN | extra,
+                    comm=comm, prec="f32")
+        assert fw.ok and bw.ok
+        dx.upload(x32)
+        for _ in range(warmup):
+            fw.execute()
+            bw.execute()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fw.execute()
+            bw.execute()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        fw.execute()
+        kf = dk.download(np.float32, 2 * mem)
+        bw.execute()
+        y = dy.download(np.float32, N).astype(np.float64)
+        passes = []
+        for pname, pl, a, b in (("r2c", fw, dx, dk), ("c2r", bw, dk, dy)):
+            npass = L.pfft_amd_plan_npasses(pl.handle)
+            msv = (ctypes.c_double * npass)()
+            byv = (ctypes.c_double * npass)()
+            assert L.pfft_amd_time_passes(pl.handle, a.ptr, b.ptr, 10, msv, byv) == 0
+            lines = pl.describe()
+            for k in range(npass):
+                passes.append({"plan": pname, "pass": k, "ms": round(msv[k], 4),
+                               "GBps": round(byv[k] / (msv[k] * 1e-3) / 1e9, 1),
+                               "kernel": lines[k].split("kernel=")[-1]})
+        forms.append({"form": name, "ms_per_pair": round(ms, 4),
+                      "speedup_vs_fp64": round(ms64 / ms, 3),
+                      "roundtrip_max_err": float(np.max(np.abs(y / N - x))),
+                      "passes": passes, "_forward": kf})
+        for o in (fw, bw):
+            o.destroy()
+        for b in (dx, dk, dy):
+            b.free()
+    return forms
+
+
+def pair_form_errors(forms, ref, ref_name):
+    """rel-L2 and max error (relative to the largest magnitude) of each
+    form's forward output against the FP64 reference `ref` (flat)."""
+    den = float(np.sqrt(np.dot(ref, ref)))
+    mx = float(np.max(np.abs(ref)))
+    for f in forms:
+        k = f.pop("_forward").astype(np.float64)
+        d = k - ref
+        f["rel_l2_fwd"] = float(np.sqrt(np.dot(d, d)) / den)
+        f["max_err_fwd"] = float(np.max(np.abs(d)) / mx)
+    target = [f for f in forms if f["form"] == "fp32 mfma radix-16"][0]
+    return {"workload": "the same pair in complex64 (input rounded to FP32): %s" % (
+                "plain, FP32 MFMA radix-16 stages, split-BF16 MFMA"),
+            "reference": ref_name, "forms": forms,
+            "target": {"form": target["form"], "speedup_vs_fp64": target["speedup_vs_fp64"],
+                       "rel_l2_fwd": target["rel_l2_fwd"], "goal": ">= 2x at <= 1e-5",
+                       "met": target["speedup_vs_fp64"] >= 2.0 and target["rel_l2_fwd"] <= 1e-5}}
 
 
 def mixed_summary(r):
@@ -226,6 +308,8 @@ def main():
 
     y = dy.download(np.float64, N)
     rt_err = float(np.max(np.abs(y / N - x)))
+    fw.execute()
+    k64 = dk.download(np.float64, 2 * mem)
 
     # per-pass durations (HIP events on the library stream)
     passes = []
@@ -296,13 +380,23 @@ def main():
     cores, cpu_desc = host_cpu()
     if args.cpu_cores:
         cores = args.cpu_cores
+    forms = None
     if not args.no_mixed:
+        forms = pair_forms(L, n, x, args.steps, args.warmup, ms, k64)
         import bench_configs
-        out["mixed"] = mixed_summary(bench_configs.cfg3_forms(args.seed, steps=5, cores=cores))
+        out["precision_study"] = mixed_summary(
+            bench_configs.cfg3_forms(args.seed, steps=5, cores=cores))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, cores, cpu_desc, args.seed)
     else:
         out["cpu_baseline"] = None
+    if forms is not None:
+        if _ORACLE_FORWARD is not None:
+            ref, ref_name = _ORACLE_FORWARD, "the FP64 oracle's forward output (CPU leg)"
+        else:
+            ref, ref_name = k64, ("the FP64 path's forward output (within 1e-12 of the oracle "
+                                  "at this size, tests/test_gpu_parity.py)")
+        out["mixed"] = pair_form_errors(forms, ref, ref_name)
     print(json.dumps(out))
 
 

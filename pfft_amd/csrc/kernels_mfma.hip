@@ -663,6 +663,11 @@ static int launch_mfma_nf(const pfk_pass *p, const void *src, void *dst) {
 int mfma_covers(const pfk_pass *p) {
   if (p->n1 || p->dec_in || p->dec_out || p->twn) return 0;   // steps of long lines
   if (p->mfma == 0 || p->prec != PFK_F32 || p->kind != PFK_C2C || p->hw) return 0;
+  // the MFMA kernels coalesce lines that are contiguous on both sides or
+  // adjacent along the tile dimension on both sides; a transposing pass
+  // (one of each) runs the vector-ALU kernels, which stage one side
+  // through LDS
+  if (!((p->sin == 1 && p->sout == 1) || (p->tsin == 1 && p->tsout == 1))) return 0;
   return p->nf >= 256 && p->nf <= 4096 && (p->nf & (p->nf - 1)) == 0;
 }
 

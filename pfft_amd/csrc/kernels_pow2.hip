@@ -203,14 +203,19 @@ __device__ __forceinline__ void stage(C2<T> *v, int *pos, const int t,
 // padding.  Line-major lane order (consecutive lanes = consecutive
 // positions of one line): [line][position] with one slot of padding per 16
 // positions, so the radix-8/16 output scatters (position strides 8 and 16)
-// spread over the banks.
+// spread over the banks, and an odd row pitch, so that the tile-major
+// accesses of a staged load or store (consecutive lanes = consecutive
+// lines at one position) do too.  The row has room for position NF (the
+// Nyquist bin of c2r).
+template <int NF>
+constexpr int lm_pitch() { return NF + NF / 16 + 1; }
 template <int NF, int C, bool LM>
 __device__ __forceinline__ int lidx(int p, int c) {
-  if constexpr (LM) return c * (NF + NF / 16) + p + (p >> 4);
+  if constexpr (LM) return c * lm_pitch<NF>() + p + (p >> 4);
   else return p * C + c;
 }
 template <int NF, int C, bool LM>
-constexpr int lds_slots() { return LM ? C * (NF + NF / 16) : NF * C; }
+constexpr int lds_slots() { return LM ? C * lm_pitch<NF>() : NF * C; }
 
 // Exchange through LDS: write v[m] at position pos[m], read back t + T*m.
 template <typename T, int NF, int C, int E, bool LM>
@@ -370,6 +375,13 @@ __device__ __forceinline__ void load_c2c_pl(const pfk_pass &P, const T *src, con
 // LM: line-major lane order (thread = c*T + t: a wave's lanes run along a
 // line), for passes whose lines are contiguous in memory; otherwise
 // thread = t*C + c (a wave's lanes run across the tile's lines).
+// XS (line-major plain passes only): 1 = the loads, 2 = the stores take the
+// tile-major lane order and go through the LDS tile.  This is for a
+// transposing pass whose lines are contiguous on one side and adjacent
+// along the tile dimension on the other: complex64 has 8-byte elements,
+// so tile-major lanes move 128-byte segments across 16 lines, and
+// line-major lanes move 512-byte runs along one line.  Each side is
+// accessed in the order that coalesces it.
 // TW: output k of tile line c0 + c is multiplied by W_twn^((c0+c)*k) from
 // the two-level table tw2 (first step of a long line, pfk.h).
 // PL: plain pass (every input and output position present, no
@@ -377,7 +389,7 @@ __device__ __forceinline__ void load_c2c_pl(const pfk_pass &P, const T *src, con
 // uniform stride, which halves the vector instructions spent per element
 // outside the butterflies.
 template <typename T, int NF, int C, int E, int SIGN, bool LM = false, bool TW = false,
-          bool PL = false>
+          bool PL = false, int XS = 0>
 __global__ void __launch_bounds__(C *NF / E)
 k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
            const T *__restrict__ src, T *__restrict__ dst) {
@@ -385,8 +397,10 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
   constexpr bool TWL = tw_in_lds<T>(lds_slots<NF, C, LM>(), NF, 0);
   __shared__ C2<T> lds[lds_slots<NF, C, LM>()];
   __shared__ C2<T> twl[TWL ? NF : 1];
+  static_assert(XS == 0 || (LM && PL && !TW), "staged loads or stores: line-major plain passes");
   const int tid = threadIdx.x;
   const int c0 = LM ? tid / TT : tid % C, t0 = LM ? tid % TT : tid / C;
+  const int cx0 = tid % C, tx0 = tid / C;   // tile-major lane order of staged accesses
   const C2<T> *twg = reinterpret_cast<const C2<T> *>(P.tw);
   if constexpr (TWL) stage_table(twl, twg, NF);
   const C2<T> *tw = TWL ? twl : twg;
@@ -399,13 +413,22 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
   auto step = [&](C2<T> *cur, C2<T> *nxt) -> bool {
     // lane indices made opaque per step: index arithmetic is recomputed in
     // the loop instead of being hoisted and kept live in registers
-    int t = t0, c = c0;
-    asm volatile("" : "+v"(t), "+v"(c));
+    int t = t0, c = c0, tx = tx0, cx = cx0;
+    asm volatile("" : "+v"(t), "+v"(c), "+v"(tx), "+v"(cx));
     const int64_t ni = item + gridDim.x;
     const bool more = ni < item_end;
     const TileCtx tn = tile_at(P, C, more ? ni : item, ntiles);
-    if constexpr (PL) load_c2c_pl<T, NF, C, E>(P, src, tn, more, c, t, nxt);
+    if constexpr (XS == 1) load_c2c_pl<T, NF, C, E>(P, src, tn, more, cx, tx, nxt);
+    else if constexpr (PL) load_c2c_pl<T, NF, C, E>(P, src, tn, more, c, t, nxt);
     else load_c2c<T, NF, C, E>(P, src, tn, more, c, t, nxt);
+    if constexpr (XS == 1) {   // tile-major loads -> line-major registers
+#pragma unroll
+      for (int m = 0; m < E; ++m) lds[lidx<NF, C, LM>(tx + TT * m, cx)] = cur[m];
+      lds_barrier();
+#pragma unroll
+      for (int m = 0; m < E; ++m) cur[m] = lds[lidx<NF, C, LM>(t + TT * m, c)];
+      lds_barrier();
+    }
     const bool valid = c < tc.nl;
     if constexpr (!PL) {
 #pragma unroll
@@ -417,7 +440,20 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       }
     }
     fft_regs<T, NF, C, E, SIGN, LM>(cur, lds, t, c, tw);
-    if (PL && valid) {
+    if constexpr (XS == 2) {   // line-major registers -> tile-major stores
+#pragma unroll
+      for (int m = 0; m < E; ++m) lds[lidx<NF, C, LM>(t + TT * m, c)] = cur[m];
+      lds_barrier();
+#pragma unroll
+      for (int m = 0; m < E; ++m) cur[m] = lds[lidx<NF, C, LM>(tx + TT * m, cx)];
+      if (cx < tc.nl) {
+        C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)cx * P.tsout +
+                   (int64_t)tx * P.sout;
+        const int64_t st = (int64_t)TT * P.sout;
+#pragma unroll
+        for (int m = 0; m < E; ++m) st_stream(d + m * st, cur[m]);
+      }
+    } else if (PL && valid) {
       // lines are independent, so a ragged tile's unused lines (garbage in,
       // never stored) cannot affect the stored ones
       C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout +
@@ -449,7 +485,8 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
     return true;
   };
   C2<T> va[E], vb[E];
-  if constexpr (PL) load_c2c_pl<T, NF, C, E>(P, src, tc, true, c0, t0, va);
+  if constexpr (XS == 1) load_c2c_pl<T, NF, C, E>(P, src, tc, true, cx0, tx0, va);
+  else if constexpr (PL) load_c2c_pl<T, NF, C, E>(P, src, tc, true, c0, t0, va);
   else load_c2c<T, NF, C, E>(P, src, tc, true, c0, t0, va);
   if constexpr (TWL) lds_barrier();
   while (step(va, vb) && step(vb, va)) {
@@ -472,17 +509,17 @@ __device__ __forceinline__ void load_r2c(const pfk_pass &P, const T *src, const 
   for (int m = 0; m < E; ++m) v[m] = ld_stream(s + (live ? t : 0) + step * m);
 }
 
-template <typename T, int NF, int C, int E, int SIGN, bool PL = false>
+template <typename T, int NF, int C, int E, int SIGN, bool PL = false, bool LM = false>
 __global__ void __launch_bounds__(C *NF / E)
 k_r2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
            const T *__restrict__ src, T *__restrict__ dst) {
   constexpr int TT = NF / E;
   constexpr int H = NF;  // h = n/2
-  constexpr bool TWL = tw_in_lds<T>(NF * C, NF, NF);
-  __shared__ C2<T> lds[NF * C];
+  constexpr bool TWL = tw_in_lds<T>(lds_slots<NF, C, LM>(), NF, NF);
+  __shared__ C2<T> lds[lds_slots<NF, C, LM>()];
   __shared__ C2<T> twl[TWL ? NF : 1], rtwl[TWL ? NF : 1];
   const int tid = threadIdx.x;
-  const int c0 = tid % C, t0 = tid / C;
+  const int c0 = LM ? tid / TT : tid % C, t0 = LM ? tid % TT : tid / C;
   const C2<T> *twg = reinterpret_cast<const C2<T> *>(P.tw);
   const C2<T> *rtwg = reinterpret_cast<const C2<T> *>(P.rtw);
   if constexpr (TWL) {
@@ -510,10 +547,10 @@ k_r2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
         cur[m] = valid ? C2<T>{cur[m].x * fx, cur[m].y * fy} : C2<T>{0, 0};
       }
     }
-    fft_regs<T, NF, C, E, SIGN>(cur, lds, t, c, tw);
+    fft_regs<T, NF, C, E, SIGN, LM>(cur, lds, t, c, tw);
     // mirror exchange: Z_{h-k}
 #pragma unroll
-    for (int m = 0; m < E; ++m) lds[(t + TT * m) * C + c] = cur[m];
+    for (int m = 0; m < E; ++m) lds[lidx<NF, C, LM>(t + TT * m, c)] = cur[m];
     lds_barrier();
     if (PL && valid) {
       // every output k = t + T*m (and the Nyquist bin, thread t = 0) is kept
@@ -524,7 +561,7 @@ k_r2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       for (int m = 0; m < E; ++m) {
         const int k = t + TT * m;
         const C2<T> a = cur[m];
-        const C2<T> b = lds[((H - k) & (H - 1)) * C + c];
+        const C2<T> b = lds[lidx<NF, C, LM>((H - k) & (H - 1), c)];
         const C2<T> Ev{(a.x + b.x) * (T)0.5, (a.y - b.y) * (T)0.5};
         const C2<T> O{(a.y + b.y) * (T)0.5, -(a.x - b.x) * (T)0.5};
         const C2<T> WO = cmul(O, rtw[k]);
@@ -538,7 +575,7 @@ k_r2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       for (int m = 0; m < E; ++m) {
         const int k = t + TT * m;
         const C2<T> a = cur[m];
-        const C2<T> b = lds[((H - k) & (H - 1)) * C + c];
+        const C2<T> b = lds[lidx<NF, C, LM>((H - k) & (H - 1), c)];
         // Ev = (a + conj b)/2, O = -i (a - conj b)/2, X = Ev + W^k O
         const C2<T> Ev{(a.x + b.x) * (T)0.5, (a.y - b.y) * (T)0.5};
         const C2<T> O{(a.y + b.y) * (T)0.5, -(a.x - b.x) * (T)0.5};
@@ -598,17 +635,18 @@ __device__ __forceinline__ void load_c2r_pl(const pfk_pass &P, const T *src, con
   xh = ld_stream(s + (live ? (int64_t)NF * P.sin : 0));
 }
 
-template <typename T, int NF, int C, int E, int SIGN, bool PL = false>
+template <typename T, int NF, int C, int E, int SIGN, bool PL = false, bool LM = false>
 __global__ void __launch_bounds__(C *NF / E)
 k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
            const T *__restrict__ src, T *__restrict__ dst) {
   constexpr int TT = NF / E;
   constexpr int H = NF;
-  constexpr bool TWL = tw_in_lds<T>((NF + 1) * C, NF, NF);
-  __shared__ C2<T> lds[(NF + 1) * C];
+  constexpr int SLOTS = LM ? lds_slots<NF, C, true>() : (NF + 1) * C;   // with the Nyquist bin
+  constexpr bool TWL = tw_in_lds<T>(SLOTS, NF, NF);
+  __shared__ C2<T> lds[SLOTS];
   __shared__ C2<T> twl[TWL ? NF : 1], rtwl[TWL ? NF : 1];
   const int tid = threadIdx.x;
-  const int c0 = tid % C, t0 = tid / C;
+  const int c0 = LM ? tid / TT : tid % C, t0 = LM ? tid % TT : tid / C;
   const C2<T> *twg = reinterpret_cast<const C2<T> *>(P.tw);
   const C2<T> *rtwg = reinterpret_cast<const C2<T> *>(P.rtw);
   if constexpr (TWL) {
@@ -632,8 +670,8 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
     // X_k, k = t + T*m, and X_h (kept by thread t == 0) into LDS
     if constexpr (PL) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) lds[(t + TT * m) * C + c] = cur[m];
-      if (t == 0) lds[H * C + c] = C2<T>{curh.x, (T)0};
+      for (int m = 0; m < E; ++m) lds[lidx<NF, C, LM>(t + TT * m, c)] = cur[m];
+      if (t == 0) lds[lidx<NF, C, LM>(H, c)] = C2<T>{curh.x, (T)0};
     } else {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -643,12 +681,12 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
         T f = msign<T>(P.imod, j);
         if (P.hw == 1 && k != 0) f *= (T)0.5;
         cur[m] = ok ? C2<T>{cur[m].x * f, cur[m].y * f} : C2<T>{0, 0};
-        lds[k * C + c] = cur[m];
+        lds[lidx<NF, C, LM>(k, c)] = cur[m];
       }
       if (t == 0) {
         const int jh = H - zin;
         const bool okh = valid && jh >= 0 && jh < nin;
-        lds[H * C + c] = okh ? C2<T>{curh.x * msign<T>(P.imod, jh), (T)0} : C2<T>{0, 0};
+        lds[lidx<NF, C, LM>(H, c)] = okh ? C2<T>{curh.x * msign<T>(P.imod, jh), (T)0} : C2<T>{0, 0};
       }
     }
     lds_barrier();
@@ -659,10 +697,10 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       const C2<T> a = cur[m];
       C2<T> z;
       if (k == 0) {
-        const T xhr = lds[H * C + c].x;
+        const T xhr = lds[lidx<NF, C, LM>(H, c)].x;
         z = C2<T>{a.x + xhr, a.x - xhr};
       } else {
-        const C2<T> b = lds[(H - k) * C + c];
+        const C2<T> b = lds[lidx<NF, C, LM>(H - k, c)];
         const C2<T> Ev{a.x + b.x, a.y - b.y};
         const C2<T> O = cmul(C2<T>{a.x - b.x, a.y + b.y}, rtw[k]);
         z = C2<T>{Ev.x - O.y, Ev.y + O.x};
@@ -670,7 +708,7 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       cur[m] = z;
     }
     lds_barrier();
-    fft_regs<T, NF, C, E, SIGN>(cur, lds, t, c, tw);
+    fft_regs<T, NF, C, E, SIGN, LM>(cur, lds, t, c, tw);
     if (valid) {
       C2<T> *d = reinterpret_cast<C2<T> *>(dst + tc.db + (int64_t)c * P.tsout);
 #pragma unroll
@@ -746,10 +784,20 @@ static bool pass_plain(const pfk_pass *p) {
   return false;
 }
 
+// Line-major staging of a complex64 c2c pass (the XS parameter of
+// k_c2c_pow2): 1 when the lines are contiguous only on the store side and
+// adjacent along the tile dimension on the load side, 2 for the reverse.
+static int lm_staging(const pfk_pass *p) {
+  if (p->sin != 1 && p->tsin == 1 && p->sout == 1) return 1;
+  if (p->sin == 1 && p->sout != 1 && p->tsout == 1) return 2;
+  return 0;
+}
+
 template <typename T, int NF, int C, int E, int SIGN, bool TWOK, bool PL>
 static int launch_kind_pl(const pfk_pass *p, int64_t ntiles, const void *src, void *dst) {
   constexpr int NT = C * NF / E;
-  static int64_t grid_c2c = 0, grid_lm = 0, grid_r2c = 0, grid_c2r = 0, grid_tw = 0;
+  static int64_t grid_c2c = 0, grid_lm = 0, grid_lm1 = 0, grid_lm2 = 0, grid_r2c = 0,
+                 grid_c2r = 0, grid_tw = 0, grid_r2c_lm = 0, grid_c2r_lm = 0;
   const bool persistent = !(p->variant & 4);
   const T *s = (const T *)src;
   T *d = (T *)dst;
@@ -761,9 +809,27 @@ static int launch_kind_pl(const pfk_pass *p, int64_t ntiles, const void *src, vo
     }
     return 1;
   }
+  // line-major lane order (variant bit 8): complex64 only
+  constexpr bool F32 = sizeof(T) == 4;
+  const bool lm = F32 && (p->variant & 8);
   if (p->kind == PFK_C2C) {
-    if constexpr (sizeof(T) == 4) {   // line-major lane order: complex64 only
-      if (p->variant & 8) {
+    if constexpr (F32) {
+      if (lm) {
+        const int xs = PL ? lm_staging(p) : 0;
+        if constexpr (PL) {
+          if (xs == 1) {
+            if (!grid_lm1)
+              grid_lm1 = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, true, false, true, 1>, NT);
+            return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, true, false, true, 1>, p, ntiles,
+                                grid_lm1, persistent, NT, s, d);
+          }
+          if (xs == 2) {
+            if (!grid_lm2)
+              grid_lm2 = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, true, false, true, 2>, NT);
+            return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, true, false, true, 2>, p, ntiles,
+                                grid_lm2, persistent, NT, s, d);
+          }
+        }
         if (!grid_lm) grid_lm = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, true, false, PL>, NT);
         return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, true, false, PL>, p, ntiles, grid_lm,
                             persistent, NT, s, d);
@@ -773,9 +839,23 @@ static int launch_kind_pl(const pfk_pass *p, int64_t ntiles, const void *src, vo
     return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, false, false, PL>, p, ntiles, grid_c2c,
                         persistent, NT, s, d);
   } else if (p->kind == PFK_R2C) {
+    if constexpr (F32) {
+      if (lm) {
+        if (!grid_r2c_lm) grid_r2c_lm = persistent_grid(k_r2c_pow2<T, NF, C, E, SIGN, PL, true>, NT);
+        return launch_items(k_r2c_pow2<T, NF, C, E, SIGN, PL, true>, p, ntiles, grid_r2c_lm,
+                            persistent, NT, s, d);
+      }
+    }
     if (!grid_r2c) grid_r2c = persistent_grid(k_r2c_pow2<T, NF, C, E, SIGN, PL>, NT);
     return launch_items(k_r2c_pow2<T, NF, C, E, SIGN, PL>, p, ntiles, grid_r2c, persistent, NT,
                         s, d);
+  }
+  if constexpr (F32) {
+    if (lm) {
+      if (!grid_c2r_lm) grid_c2r_lm = persistent_grid(k_c2r_pow2<T, NF, C, E, SIGN, PL, true>, NT);
+      return launch_items(k_c2r_pow2<T, NF, C, E, SIGN, PL, true>, p, ntiles, grid_c2r_lm,
+                          persistent, NT, s, d);
+    }
   }
   if (!grid_c2r) grid_c2r = persistent_grid(k_c2r_pow2<T, NF, C, E, SIGN, PL>, NT);
   return launch_items(k_c2r_pow2<T, NF, C, E, SIGN, PL>, p, ntiles, grid_c2r, persistent, NT, s,

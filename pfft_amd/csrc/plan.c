@@ -1092,7 +1092,7 @@ static const char *kernel_name(const pfk_pass *ps) {
   if (ps->n1) return "long-two-step";
   if (ps->r2r_path) return "r2r-inner-fft";
   const int fam = pfk_pass_kernels(ps);
-  const int lm = (ps->variant & 8) && ps->kind == PFK_C2C && ps->prec == PFK_F32;
+  const int lm = (ps->variant & 8) && ps->kind != PFK_R2R && ps->prec == PFK_F32;
   if (fam & PFK_HAS_MFMA) {
     static const char *nm[2][2] = {{"mfma-f32", "mfma-f32-lm"}, {"mfma-bf16x2", "mfma-bf16x2-lm"}};
     return nm[ps->mfma == 2][lm && ps->nf <= 1024];

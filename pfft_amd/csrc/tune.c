@@ -17,8 +17,8 @@
  *   (2,0) (2,1) power-of-two register kernel, half the tile width;
  *   (4,r) (6,r) as (0,r) (2,r) with one workgroup per tile instead of
  *         the persistent grid;
- *   (8,r) (12,r) as (0,r) (4,r) with line-major lane order (complex64 c2c
- *         passes whose lines are contiguous in memory);
+ *   (8,r) (12,r) as (0,r) (4,r) with line-major lane order (complex64
+ *         passes whose lines are contiguous on at least one side);
  *   (3,0) short-length kernel, one line per lane;
  *   (1,0) the general kernel.
  * A pass with one candidate is not timed.  All candidates compute the
@@ -218,8 +218,15 @@ int pa_tune_measure(pfk_pass *ps, const void *src, void *dst, unsigned effort) {
   const int reps = (effort & PFFT_EXHAUSTIVE) ? 10 : (effort & PFFT_PATIENT) ? 6 : 3;
   static const int cand[14][2] = {{0, 0}, {0, 1}, {2, 0},  {2, 1},  {4, 0},  {4, 1},  {6, 0},
                                   {6, 1}, {8, 0}, {8, 1}, {12, 0}, {12, 1}, {3, 0}, {1, 0}};
-  /* line-major lane order: complex64 c2c passes whose lines are contiguous */
-  const int lm_ok = ps->kind == PFK_C2C && ps->prec == PFK_F32 && ps->sin == 1;
+  /* line-major lane order (complex64): c2c passes whose lines are
+   * contiguous on one side and contiguous or adjacent along the tile on the
+   * other (the kernels stage a tile-adjacent side through LDS), and real
+   * passes whose complex side is contiguous too */
+  const int lm_ok =
+      ps->prec == PFK_F32 &&
+      ((ps->kind == PFK_C2C && (ps->sin == 1 || ps->sout == 1) && (ps->sin == 1 || ps->tsin == 1) &&
+        (ps->sout == 1 || ps->tsout == 1)) ||
+       (ps->kind == PFK_R2C && ps->sout == 1) || (ps->kind == PFK_C2R && ps->sin == 1));
   const int fam = pfk_pass_kernels(ps);
   void *e0 = NULL, *e1 = NULL;
   if (pfk_event_create(&e0) || pfk_event_create(&e1)) {

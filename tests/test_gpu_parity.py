@@ -251,6 +251,21 @@ def test_mfma_forms(n, flags, form, tol):
         assert err <= tol, (n, flags, form, sign, err)
 
 
+@pytest.mark.parametrize("form,tol", [(P.AMD_MFMA_F32, 5e-6), (P.AMD_MFMA_BF16X2, 1e-5)])
+@pytest.mark.parametrize("kind,flags", [("r2c", TO), ("c2r", TI)])
+def test_mfma_forms_real_pair(kind, flags, form, tol):
+    """the headline pair's shape (r2c with transposed output, c2r with
+    transposed input) in the MFMA forms at a reduced size: the column c2c
+    pass runs an MFMA kernel, the transposing c2c pass and the real pass the
+    vector-ALU kernels; both directions against the oracle"""
+    n = [256, 256, 128]
+    spec = spec_of(kind, n, flags=flags, sign=(+1 if kind == "c2r" else -1))
+    x = random_input(spec, 9).astype(np.float32).astype(np.float64)
+    got = run_gpu(spec, x, prec="f32", plan_flags_extra=form)
+    want = oracle_out(spec, x)
+    assert rel_l2(got, want, out_mask(spec)) <= tol
+
+
 @pytest.mark.parametrize("n", [[1024, 16, 512], [256, 32, 1024], [512, 8, 256]])
 def test_mfma_forms_precision_input(n):
     """the precision configuration's input (complex64 U[-1,1)^2 plus a

@@ -21,6 +21,12 @@ CASES = [
     ("c2r", [128, 64, 512], TI, "f64"),
     ("c2c", [1024, 8, 16], 0, "f32"),
     ("r2c", [32, 16, 2048], 0, "f32"),
+    # complex64 transposing passes (line-major variants stage the
+    # tile-adjacent side through LDS) and real passes with a contiguous
+    # complex side (line-major real kernels)
+    ("r2c", [64, 32, 512], TO, "f32"),
+    ("c2r", [64, 32, 512], TI, "f32"),
+    ("c2c", [128, 64, 256], TO, "f32"),
 ]
 
 
@@ -75,7 +81,7 @@ def _expected(line, prec, variant, remap):
         return "small"
     if pow2:
         name = {0: "fast", 2: "fast-narrow", 3: "fast", 4: "fast-grid", 6: "fast-narrow-grid"}
-        lm = (variant & 8) and kind == "c2c" and prec == "f32"
+        lm = (variant & 8) and kind != "r2r" and prec == "f32"
         return name[variant & 7] + ("-lm" if lm else "") + ("+xcd" if remap else "")
     return "small" if small else "general"
 
