@@ -293,6 +293,7 @@ k_mfma_tile(const pfk_pass P, const int64_t item0, const int64_t item_end, const
     const bool more = ni < item_end;
     const Tile16 tn = tile_at_cl(P, 16, more ? ni : item, ntiles);
     load(tn, lt, nx);
+    __builtin_amdgcn_sched_barrier(0);   // the prefetch stays ahead of the transform
     // stage 1: DFT-16 over n2, twiddle W_NF^(n1 k2), to LDS
     // (the split form in two batches of two: its operand splits need the
     // registers of a batch of four)

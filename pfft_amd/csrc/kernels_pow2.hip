@@ -256,6 +256,54 @@ __device__ __forceinline__ void fft_regs(C2<T> *v, C2<T> *lds, const int t, cons
   fft_stages<T, NF, C, E, SIGN, 1, LM>(v, pos, lds, t, c, tw);
 }
 
+// Exchanges of a length-NF FFT in radix-E stages (stages minus one).
+__host__ __device__ constexpr int n_exchanges(int nf, int e) {
+  int s = 0, l = 1;
+  while (nf / l >= e) { l *= e; ++s; }
+  if (nf / l > 1) ++s;
+  return s - 1;
+}
+
+// Exchange through the line-major LDS layout with a change of lane order:
+// v[m] is written at position pos[m] of line cw and read back as position
+// tr + T*m of line cr.
+template <typename T, int NF, int C, int E>
+__device__ __forceinline__ void exchange_x(C2<T> *v, const int *pos, C2<T> *lds, const int cw,
+                                           const int tr, const int cr) {
+  constexpr int TT = NF / E;
+#pragma unroll
+  for (int m = 0; m < E; ++m) lds[lidx<NF, C, true>(pos[m], cw)] = v[m];
+  lds_barrier();
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = lds[lidx<NF, C, true>(tr + TT * m, cr)];
+  lds_barrier();
+}
+
+// The FFT of fft_regs with a change of lane order at exchange SW (1-based):
+// stages before it run as thread (ta, ca) (position index, line), the
+// exchange writes in that order and reads in order (tb, cb), and the later
+// stages run as (tb, cb).  K counts the exchanges done.
+template <typename T, int NF, int C, int E, int SIGN, int L, int K, int SW>
+__device__ __forceinline__ void fft_stages_sw(C2<T> *v, int *pos, C2<T> *lds, const int ta,
+                                              const int ca, const int tb, const int cb,
+                                              const C2<T> *__restrict__ tw) {
+  constexpr int REM = NF / L;
+  constexpr bool BEFORE = K < SW;
+  const int t = BEFORE ? ta : tb;
+  const int c = BEFORE ? ca : cb;
+  if constexpr (REM >= E) {
+    constexpr bool LAST = (REM == E);
+    stage<T, NF, E, E, L, SIGN, LAST>(v, pos, t, tw);
+    if constexpr (!LAST) {
+      constexpr bool AFTER = K + 1 >= SW;   // the next stage runs as (tb, cb)
+      exchange_x<T, NF, C, E>(v, pos, lds, c, AFTER ? tb : ta, AFTER ? cb : ca);
+      fft_stages_sw<T, NF, C, E, SIGN, L * E, K + 1, SW>(v, pos, lds, ta, ca, tb, cb, tw);
+    }
+  } else if constexpr (REM > 1) {
+    stage<T, NF, E, REM, L, SIGN, true>(v, pos, t, tw);
+  }
+}
+
 struct TileCtx {
   int64_t sb, db;
   int64_t c0;   // first line of the tile along the tile dimension
@@ -372,11 +420,58 @@ __device__ __forceinline__ void load_c2c_pl(const pfk_pass &P, const T *src, con
   for (int m = 0; m < E; ++m) v[m] = ld_stream(s + m * st);
 }
 
+// Paired-line stores for complex64 tiles whose lines are adjacent in
+// memory (tile stride 1), tile-major lane order (line c = 2j + e of the
+// tile in lane parity e).  Every 16-byte store covers two adjacent lines at
+// one position: lane e = 0 of a pair writes the even element slots m = 0,
+// 2, 4, ... of lines 2j and 2j+1, lane e = 1 the odd ones, after one DPP
+// lane swap (quad_perm 1,0,3,2: lane ^ 1) per value has handed each lane
+// its partner's values.  A wave instruction then moves 128-byte segments
+// of 8 positions (the 4 of 8-byte lanes otherwise), the access shape of
+// the complex128 tiles.  An odd last line is stored with 8-byte stores.
+__device__ __forceinline__ float lane_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+
+typedef float pf4 __attribute__((ext_vector_type(4)));
+
+// stores of lines c < tc.nl: pairs as 16-byte stores, an odd last line with
+// 8-byte stores
+template <int NF, int C, int E>
+__device__ __forceinline__ void store_c2c_pair(const pfk_pass &P, float *dst, const TileCtx &tc,
+                                               const int c, const int t, const C2<float> *v) {
+  constexpr int TT = NF / E;
+  const int e = c & 1, j = c >> 1;
+  C2<float> send[E / 2], recv[E / 2];
+#pragma unroll
+  for (int i = 0; i < E / 2; ++i) {
+    send[i] = e ? v[2 * i] : v[2 * i + 1];
+    recv[i] = C2<float>{lane_xor1(send[i].x), lane_xor1(send[i].y)};
+  }
+  C2<float> *d0 = reinterpret_cast<C2<float> *>(dst) + tc.db;
+  if (2 * j + 1 < tc.nl) {
+    C2<float> *d = d0 + 2 * j + (int64_t)(t + TT * e) * P.sout;
+    const int64_t st = (int64_t)2 * TT * P.sout;
+#pragma unroll
+    for (int i = 0; i < E / 2; ++i) {
+      const C2<float> lo = e ? recv[i] : v[2 * i], hi = e ? v[2 * i + 1] : recv[i];
+      __builtin_nontemporal_store(pf4{lo.x, lo.y, hi.x, hi.y},
+                                  reinterpret_cast<pf4 *>(d + i * st));
+    }
+  } else if (e == 0 && 2 * j < tc.nl) {
+    C2<float> *d = d0 + 2 * j + (int64_t)t * P.sout;
+#pragma unroll
+    for (int m = 0; m < E; ++m) st_stream(d + (int64_t)TT * m * P.sout, v[m]);
+  }
+}
+
 // LM: line-major lane order (thread = c*T + t: a wave's lanes run along a
 // line), for passes whose lines are contiguous in memory; otherwise
 // thread = t*C + c (a wave's lanes run across the tile's lines).
 // XS (line-major plain passes only): 1 = the loads, 2 = the stores take the
-// tile-major lane order and go through the LDS tile.  This is for a
+// tile-major lane order; the FFT switches between the two orders at its
+// first (1) or last (2) LDS exchange, so the staging costs no extra pass
+// through LDS.  This is for a
 // transposing pass whose lines are contiguous on one side and adjacent
 // along the tile dimension on the other: complex64 has 8-byte elements,
 // so tile-major lanes move 128-byte segments across 16 lines, and
@@ -389,7 +484,7 @@ __device__ __forceinline__ void load_c2c_pl(const pfk_pass &P, const T *src, con
 // uniform stride, which halves the vector instructions spent per element
 // outside the butterflies.
 template <typename T, int NF, int C, int E, int SIGN, bool LM = false, bool TW = false,
-          bool PL = false, int XS = 0>
+          bool PL = false, int XS = 0, int PAIR = 0>
 __global__ void __launch_bounds__(C *NF / E)
 k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
            const T *__restrict__ src, T *__restrict__ dst) {
@@ -398,6 +493,9 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
   __shared__ C2<T> lds[lds_slots<NF, C, LM>()];
   __shared__ C2<T> twl[TWL ? NF : 1];
   static_assert(XS == 0 || (LM && PL && !TW), "staged loads or stores: line-major plain passes");
+  // PAIR: paired-line stores (complex64 plain passes, tile-major side)
+  static_assert(PAIR == 0 || (sizeof(T) == 4 && PL && !TW && C % 2 == 0), "paired lines");
+  static_assert(!PAIR || !LM || XS == 2, "paired stores need tile-major stores");
   const int tid = threadIdx.x;
   const int c0 = LM ? tid / TT : tid % C, t0 = LM ? tid % TT : tid / C;
   const int cx0 = tid % C, tx0 = tid / C;   // tile-major lane order of staged accesses
@@ -421,14 +519,6 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
     if constexpr (XS == 1) load_c2c_pl<T, NF, C, E>(P, src, tn, more, cx, tx, nxt);
     else if constexpr (PL) load_c2c_pl<T, NF, C, E>(P, src, tn, more, c, t, nxt);
     else load_c2c<T, NF, C, E>(P, src, tn, more, c, t, nxt);
-    if constexpr (XS == 1) {   // tile-major loads -> line-major registers
-#pragma unroll
-      for (int m = 0; m < E; ++m) lds[lidx<NF, C, LM>(tx + TT * m, cx)] = cur[m];
-      lds_barrier();
-#pragma unroll
-      for (int m = 0; m < E; ++m) cur[m] = lds[lidx<NF, C, LM>(t + TT * m, c)];
-      lds_barrier();
-    }
     const bool valid = c < tc.nl;
     if constexpr (!PL) {
 #pragma unroll
@@ -439,13 +529,23 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
         cur[m] = ok ? C2<T>{cur[m].x * f, cur[m].y * f} : C2<T>{0, 0};
       }
     }
-    fft_regs<T, NF, C, E, SIGN, LM>(cur, lds, t, c, tw);
-    if constexpr (XS == 2) {   // line-major registers -> tile-major stores
-#pragma unroll
-      for (int m = 0; m < E; ++m) lds[lidx<NF, C, LM>(t + TT * m, c)] = cur[m];
-      lds_barrier();
-#pragma unroll
-      for (int m = 0; m < E; ++m) cur[m] = lds[lidx<NF, C, LM>(tx + TT * m, cx)];
+    if constexpr (XS == 0) {
+      fft_regs<T, NF, C, E, SIGN, LM>(cur, lds, t, c, tw);
+    } else {
+      // the lane order changes at one of the FFT's own exchanges: the first
+      // (tile-major loads, line-major stores) or the last (the reverse)
+      static_assert(n_exchanges(NF, E) >= 1, "staged passes need an exchange");
+      int pos[E];
+      if constexpr (XS == 1)
+        fft_stages_sw<T, NF, C, E, SIGN, 1, 0, 1>(cur, pos, lds, tx, cx, t, c, tw);
+      else
+        fft_stages_sw<T, NF, C, E, SIGN, 1, 0, n_exchanges(NF, E)>(cur, pos, lds, t, c, tx, cx,
+                                                                    tw);
+    }
+    if constexpr (PAIR) {   // tile-major order, paired lines (every lane takes part)
+      if constexpr (XS == 2) store_c2c_pair<NF, C, E>(P, dst, tc, cx, tx, cur);
+      else store_c2c_pair<NF, C, E>(P, dst, tc, c, t, cur);
+    } else if constexpr (XS == 2) {   // registers in tile-major order: tile-major stores
       if (cx < tc.nl) {
         C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)cx * P.tsout +
                    (int64_t)tx * P.sout;
@@ -784,6 +884,19 @@ static bool pass_plain(const pfk_pass *p) {
   return false;
 }
 
+// Paired-line stores (the PAIR parameter of k_c2c_pow2) on the store side
+// (or loads, `in`): lines adjacent in memory (tile stride 1), even element
+// strides and a 16-byte-aligned array, so that every pair of lines starts
+// on a 16-byte boundary; at least two lines.
+static bool pair_side_ok(const pfk_pass *p, bool in, const void *ptr) {
+  const int64_t ts = in ? p->tsin : p->tsout, ls = in ? p->sin : p->sout;
+  if (p->prec != PFK_F32 || ts != 1 || p->text < 2 || (ls & 1) || ((uintptr_t)ptr & 15))
+    return false;
+  for (int i = 0; i < p->nb; ++i)
+    if ((in ? p->bsin[i] : p->bsout[i]) & 1) return false;
+  return true;
+}
+
 // Line-major staging of a complex64 c2c pass (the XS parameter of
 // k_c2c_pow2): 1 when the lines are contiguous only on the store side and
 // adjacent along the tile dimension on the load side, 2 for the reverse.
@@ -797,7 +910,7 @@ template <typename T, int NF, int C, int E, int SIGN, bool TWOK, bool PL>
 static int launch_kind_pl(const pfk_pass *p, int64_t ntiles, const void *src, void *dst) {
   constexpr int NT = C * NF / E;
   static int64_t grid_c2c = 0, grid_lm = 0, grid_lm1 = 0, grid_lm2 = 0, grid_r2c = 0,
-                 grid_c2r = 0, grid_tw = 0, grid_r2c_lm = 0, grid_c2r_lm = 0;
+                 grid_c2r = 0, grid_tw = 0, grid_r2c_lm = 0, grid_c2r_lm = 0, grid_lm2p = 0;
   const bool persistent = !(p->variant & 4);
   const T *s = (const T *)src;
   T *d = (T *)dst;
@@ -817,6 +930,17 @@ static int launch_kind_pl(const pfk_pass *p, int64_t ntiles, const void *src, vo
       if (lm) {
         const int xs = PL ? lm_staging(p) : 0;
         if constexpr (PL) {
+          // paired-line stores on the tile side of a transposing pass (512-point
+          // complex64 lines: 5.10 -> 5.27 TB/s); paired loads measured no
+          // faster there and slower on passes that are tile-adjacent on both
+          // sides (5.0 -> 4.7 TB/s), so those keep 8-byte lanes
+          if (xs == 2 && pair_side_ok(p, false, dst)) {
+            if (!grid_lm2p)
+              grid_lm2p =
+                  persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, true, false, true, 2, 1>, NT);
+            return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, true, false, true, 2, 1>, p, ntiles,
+                                grid_lm2p, persistent, NT, s, d);
+          }
           if (xs == 1) {
             if (!grid_lm1)
               grid_lm1 = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, true, false, true, 1>, NT);
