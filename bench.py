@@ -44,11 +44,12 @@ def parse():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--workload", default="cfg2",
-                    choices=["cfg2", "cfg3", "cfg4", "cfg5", "gcell", "mixed"],
+                    choices=["cfg2", "cfg3", "cfg4", "cfg5", "gcell", "mixed", "pair"],
                     help="cfg2 (default): 512^3 FP64 r2c+c2r; cfg3: precision study "
                          "1024x1024x512 c2c; cfg4: 384 uneven c2c batch; cfg5: oversampled "
                          "shifted r2c forward+adjoint; gcell: ghost cells; mixed: "
-                         "mixed-radix c2c sides 240..960")
+                         "mixed-radix c2c sides 240..960; pair: the 512^3 pair in FP64 and "
+                         "the complex64 forms (no CPU legs)")
     return ap.parse_args()
 
 

@@ -345,5 +345,39 @@ def mixed(args):
     print(json.dumps(out))
 
 
+def pair(args):
+    """The 512^3 pair (r2c TRANSPOSED_OUT + c2r TRANSPOSED_IN) in FP64 and
+    the three complex64 forms, without the CPU legs (for profiler runs):
+    ms per pair and per pass, errors against the FP64 path's forward."""
+    import ctypes
+    import bench
+    L = P.lib()
+    L.pfft_amd_time_passes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]
+    L.pfft_amd_plan_npasses.argtypes = [ctypes.c_void_p]
+    n = args.n
+    N = n ** 3
+    rc, comm = P.create_procmesh([1, 1])
+    mem = P.local_size("r2c", [n] * 3, flags=P.TRANSPOSED_OUT, comm=comm)[0]
+    x = np.random.default_rng(args.seed).standard_normal(N)
+    dx, dk, dy = P.DeviceBuffer(N * 8), P.DeviceBuffer(mem * 16), P.DeviceBuffer(2 * mem * 8)
+    fw = P.Plan("r2c", [n] * 3, dx.ptr, dk.ptr, flags=P.TRANSPOSED_OUT, comm=comm)
+    bw = P.Plan("c2r", [n] * 3, dk.ptr, dy.ptr, sign=+1, flags=P.TRANSPOSED_IN, comm=comm)
+    dx.upload(x)
+    ms64 = _time(lambda: (fw.execute(), bw.execute()), args.steps, warm=args.warmup)
+    fw.execute()
+    k64 = dk.download(np.float64, 2 * mem)
+    for o in (fw, bw):
+        o.destroy()
+    for b in (dx, dk, dy):
+        b.free()
+    forms = bench.pair_forms(L, n, x, args.steps, args.warmup, ms64, k64)
+    out = bench.pair_form_errors(forms, k64, "the FP64 path's forward output")
+    out["fp64_ms_per_pair"] = round(ms64, 4)
+    print(json.dumps(out))
+
+
 def run(args):
-    {"cfg3": cfg3, "cfg4": cfg4, "cfg5": cfg5, "gcell": gcell, "mixed": mixed}[args.workload](args)
+    {"cfg3": cfg3, "cfg4": cfg4, "cfg5": cfg5, "gcell": gcell, "mixed": mixed,
+     "pair": pair}[args.workload](args)

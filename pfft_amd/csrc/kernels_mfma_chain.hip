@@ -271,16 +271,19 @@ k_mfma_tile(const pfk_pass P, const int64_t item0, const int64_t item_end, const
   Tile16 tc = tile_at_cl(P, 16, item, ntiles);
   // loads of tile `t`: lane (q, c) of wave w, group g, slot s holds
   // x[c][4w + g + Q P(q, s)]
-  auto load = [&](const Tile16 &t, int lt, cf *x) {
+  // live == false (no next tile): every lane reads the tile's first
+  // element, so the last prefetch costs one segment instead of a tile
+  auto load = [&](const Tile16 &t, int lt, cf *x, bool live) {
     const int c = lt & 15, q = (lt & 63) >> 4;
     const int w = __builtin_amdgcn_readfirstlane(lt >> 6);   // wave-uniform
-    const int cc = c < t.nl ? c : 0;
-    const cf *b = s0 + t.sb + (int64_t)cc * P.tsin + (int64_t)(4 * w) * P.sin;
+    const int cc = (live && c < t.nl) ? c : 0;
+    const int64_t sin = live ? P.sin : 0;
+    const cf *b = s0 + t.sb + (int64_t)cc * P.tsin + (int64_t)(4 * w) * sin;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const cf *bs = b + (int64_t)(Q * ptab(q, s)) * P.sin;
+      const cf *bs = b + (int64_t)(Q * ptab(q, s)) * sin;
 #pragma unroll
-      for (int g = 0; g < 4; ++g) x[4 * g + s] = ldnt(bs + (int64_t)g * P.sin);
+      for (int g = 0; g < 4; ++g) x[4 * g + s] = ldnt(bs + (int64_t)g * sin);
     }
   };
   auto step = [&](cf *x, cf *nx) -> bool {
@@ -292,7 +295,7 @@ k_mfma_tile(const pfk_pass P, const int64_t item0, const int64_t item_end, const
     const int64_t ni = item + gridDim.x;
     const bool more = ni < item_end;
     const Tile16 tn = tile_at_cl(P, 16, more ? ni : item, ntiles);
-    load(tn, lt, nx);
+    load(tn, lt, nx, more);
     __builtin_amdgcn_sched_barrier(0);   // the prefetch stays ahead of the transform
     // stage 1: DFT-16 over n2, twiddle W_NF^(n1 k2), to LDS
     // (the split form in two batches of two: its operand splits need the
@@ -380,7 +383,7 @@ k_mfma_tile(const pfk_pass P, const int64_t item0, const int64_t item_end, const
     return true;
   };
   cf xa[16], xb[16];
-  load(tc, tid, xa);
+  load(tc, tid, xa, true);
   lds_barrier();   // twiddle table
   while (step(xa, xb) && step(xb, xa)) {
   }
@@ -442,16 +445,18 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
   // lane (q, j16): first-stage butterfly rho = P(j16/4, j16%4); slot u of
   // sub-transform j of line i is x[R (rho + 16 P(q, u)) + j], held in
   // x[(i R + j) 4 + u]
-  auto load = [&](const Tile16 &t, int lt, cf *x) {
+  // live == false (no next tile): every lane reads the tile's first elements
+  auto load = [&](const Tile16 &t, int lt, cf *x, bool live) {
     const int q = (lt & 63) >> 4, j16 = lt & 15;
     const int w = __builtin_amdgcn_readfirstlane(lt >> 6);
-    const int rho = ptab(j16 >> 2, j16 & 3);
+    const int rho = live ? ptab(j16 >> 2, j16 & 3) : 0;
 #pragma unroll
     for (int i = 0; i < LPW; ++i) {
       const int ln = w * LPW + i;
-      const cf *s = s0 + t.sb + (int64_t)(ln < t.nl ? ln : 0) * P.tsin + R * rho;
+      const cf *s = s0 + t.sb + (int64_t)((live && ln < t.nl) ? ln : 0) * P.tsin + R * rho;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) load_run<R>(s + 16 * R * ptab(q, u), x + i * R * 4 + u);
+      for (int u = 0; u < 4; ++u)
+        load_run<R>(s + (live ? 16 * R * ptab(q, u) : 0), x + i * R * 4 + u);
     }
   };
   auto step = [&](cf *x, cf *nx) -> bool {
@@ -463,7 +468,7 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
     const int64_t ni = item + gridDim.x;
     const bool more = ni < item_end;
     const Tile16 tn = tile_at_cl(P, CL, more ? ni : item, ntiles);
-    load(tn, lt, nx);
+    load(tn, lt, nx, more);
     // keep the prefetch ahead of the transform (the scheduler would
     // otherwise sink the loads below it to save registers)
     __builtin_amdgcn_sched_barrier(0);
@@ -554,7 +559,7 @@ k_mfma_line(const pfk_pass P, const int64_t item0, const int64_t item_end, const
     return true;
   };
   cf xa[4 * NP], xb[4 * NP];
-  load(tc, tid, xa);
+  load(tc, tid, xa, true);
   lds_barrier();   // twiddle table
   while (step(xa, xb) && step(xb, xa)) {
   }
