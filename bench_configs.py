@@ -4,10 +4,10 @@ cfg3  precision study: 3-d c2c 1024x1024x512, complex64 input U[-1,1)^2
       plus a 1e-3-RMS 1/|k|^2 spectral component; forward+backward in FP64
       (plain), FP32 (plain Stockham) and the two MFMA forms; relative L2 /
       max error of each forward result vs an FP64 CPU reference (also part
-      of the default bench.py line, as "mixed").
+      of the default bench.py line, as "precision_study").
 cfg4  384 3-d c2c complex64 transforms, sides drawn from
-      {24,32,48,60,64,80,96,128}: batched (one persistent launch per pass
-      level, replayed hipGraph) vs a per-transform execute loop.
+      {24,32,48,60,64,80,96,128}: batched (persistent launches per pass
+      level and kernel, replayed hipGraph) vs a per-transform execute loop.
 cfg5  oversampled shifted r2c ni=128^3 -> n=256^3, loss sum w_k |X_k|^2:
       forward + adjoint (gradient) time, FP64 and FP32.
 gcell ghost-cell exchange and reduce of a 512^3 block with 4 ghost cells on
@@ -16,6 +16,9 @@ mixed 3-d c2c forward transforms of mixed-radix sides (480^3, 384^3, 240^3
       and 960x960x64) and r2c forward transforms (480^3, 384^3) in FP64 and
       complex64: per-pass times and kernels, and the relative L2 difference
       from torch.fft.fftn / rfftn of the same input.
+pair  the 512^3 r2c+c2r pair in FP64 and in the complex64 forms (FP32,
+      FP32-MFMA, split-BF16) without the CPU legs: the default line's
+      "mixed" measurement alone, for profiler runs.
 Each prints one JSON line.  Inputs are generated on the GPU with torch
 (test-data plumbing) and resident in HBM before timing.
 """

@@ -441,9 +441,9 @@ int pfft_amd_execute_batch_noalloc(const void *const *plans, int count);
 int pfft_amd_memcpy_to_device(void *dst_dev, const void *src_host, size_t bytes);
 int pfft_amd_memcpy_to_host(void *dst_host, const void *src_dev, size_t bytes);
 /* Measurement hooks: number of device passes of a plan; mean duration
- * (ms) of each pass over `iters` runs on device arrays in/out together
- * with the bytes each pass reads plus writes; a text description of the
- * passes. */
+ * (ms) of each pass over `iters` runs on device arrays in/out (after one
+ * untimed run) together with the bytes each pass reads plus writes; a text
+ * description of the passes. */
 int pfft_amd_plan_npasses(const void *plan);
 int pfft_amd_time_passes(const void *plan, void *in, void *out, int iters, double *ms,
                          double *bytes);

@@ -922,14 +922,15 @@ static int launch_kind_pl(const pfk_pass *p, int64_t ntiles, const void *src, vo
     }
     return 1;
   }
-  // line-major lane order (variant bit 8): complex64 only
-  constexpr bool F32 = sizeof(T) == 4;
-  const bool lm = F32 && (p->variant & 8);
+  // line-major lane order (variant bit 8): complex64, and complex128 on
+  // plain passes
+  constexpr bool LMOK = sizeof(T) == 4 || PL;
+  const bool lm = LMOK && (p->variant & 8);
   if (p->kind == PFK_C2C) {
-    if constexpr (F32) {
+    if constexpr (LMOK) {
       if (lm) {
         const int xs = PL ? lm_staging(p) : 0;
-        if constexpr (PL) {
+        if constexpr (PL && sizeof(T) == 4) {
           // paired-line stores on the tile side of a transposing pass (512-point
           // complex64 lines: 5.10 -> 5.27 TB/s); paired loads measured no
           // faster there and slower on passes that are tile-adjacent on both
@@ -941,6 +942,8 @@ static int launch_kind_pl(const pfk_pass *p, int64_t ntiles, const void *src, vo
             return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, true, false, true, 2, 1>, p, ntiles,
                                 grid_lm2p, persistent, NT, s, d);
           }
+        }
+        if constexpr (PL) {
           if (xs == 1) {
             if (!grid_lm1)
               grid_lm1 = persistent_grid(k_c2c_pow2<T, NF, C, E, SIGN, true, false, true, 1>, NT);
@@ -963,7 +966,7 @@ static int launch_kind_pl(const pfk_pass *p, int64_t ntiles, const void *src, vo
     return launch_items(k_c2c_pow2<T, NF, C, E, SIGN, false, false, PL>, p, ntiles, grid_c2c,
                         persistent, NT, s, d);
   } else if (p->kind == PFK_R2C) {
-    if constexpr (F32) {
+    if constexpr (LMOK) {
       if (lm) {
         if (!grid_r2c_lm) grid_r2c_lm = persistent_grid(k_r2c_pow2<T, NF, C, E, SIGN, PL, true>, NT);
         return launch_items(k_r2c_pow2<T, NF, C, E, SIGN, PL, true>, p, ntiles, grid_r2c_lm,
@@ -974,7 +977,7 @@ static int launch_kind_pl(const pfk_pass *p, int64_t ntiles, const void *src, vo
     return launch_items(k_r2c_pow2<T, NF, C, E, SIGN, PL>, p, ntiles, grid_r2c, persistent, NT,
                         s, d);
   }
-  if constexpr (F32) {
+  if constexpr (LMOK) {
     if (lm) {
       if (!grid_c2r_lm) grid_c2r_lm = persistent_grid(k_c2r_pow2<T, NF, C, E, SIGN, PL, true>, NT);
       return launch_items(k_c2r_pow2<T, NF, C, E, SIGN, PL, true>, p, ntiles, grid_c2r_lm,
@@ -1091,6 +1094,8 @@ int launch_fast(const pfk_pass *p, const void *src, void *dst) {
 }
 
 }  // namespace pfk
+
+extern "C" int pfk_pass_plain(const pfk_pass *p) { return pfk::pass_plain(p) ? 1 : 0; }
 
 extern "C" int pfk_pass_kernels(const pfk_pass *p) {
   if (pfk::g_generic_only) return 0;

@@ -277,6 +277,9 @@ int  pfk_pass_check(const pfk_pass *p);
 #define PFK_HAS_SMALL 2
 #define PFK_HAS_MFMA  4
 int  pfk_pass_kernels(const pfk_pass *p);
+/* 1 if the pass is plain (every input and output position present, no
+ * modulation or weights): the instantiations without masks run it */
+int  pfk_pass_plain(const pfk_pass *p);
 
 /* Timing helpers on the library stream (HIP events). */
 int  pfk_event_create(void **ev);

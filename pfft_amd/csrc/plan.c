@@ -1053,35 +1053,46 @@ int pa_plans_execute_batch_noalloc(const pa_plan *const *plans, int count) {
 }
 
 /* Times every pass of the plan over `iters` back-to-back executions on
- * device arrays (HIP events on the library stream between passes).
- * ms[k] receives the mean duration of pass k; bytes[k] the bytes pass k
- * reads plus writes (one sweep of its source and destination). */
+ * device arrays, after one untimed execution: one chain of HIP events on
+ * the library stream, one between every two passes, read after a single
+ * synchronisation.  (Reading the events after each execution would let the
+ * stream run dry, and the next execution's first event would then carry
+ * the host's launch latency into pass 0.)  ms[k] receives the mean
+ * duration of pass k; bytes[k] the bytes pass k reads plus writes (one
+ * sweep of its source and destination). */
 int pa_plan_time_passes(const pa_plan *p, void *in, void *out, int iters, double *ms,
                         double *bytes) {
   if (!p || iters < 1 || !pfk_is_device_ptr(in) || !pfk_is_device_ptr(out)) return 1;
   void *bufs[4] = {in, out, p->scratch[0], p->scratch[1]};
-  void *ev[PA_MAXRNK + 1];
-  for (int k = 0; k <= p->npass; k++)
-    if (pfk_event_create(&ev[k])) return 1;
-  for (int k = 0; k < p->npass; k++) ms[k] = 0;
-  int err = 0;
-  for (int it = 0; it < iters && !err; it++) {
-    pfk_event_record(ev[0]);
+  const int nev = iters * p->npass + 1;
+  void **ev = (void **)calloc((size_t)nev, sizeof(void *));
+  if (!ev) return 1;
+  int err = 0, made = 0;
+  for (; made < nev && !err; made++)
+    if (pfk_event_create(&ev[made])) err = 1;
+  for (int k = 0; k < p->npass && !err; k++)
+    err = pfk_launch_pass(&p->pass[k], bufs[p->src_role[k]], bufs[p->dst_role[k]]);
+  if (!err) err = pfk_event_record(ev[0]);
+  for (int it = 0; it < iters && !err; it++)
     for (int k = 0; k < p->npass && !err; k++) {
       err = pfk_launch_pass(&p->pass[k], bufs[p->src_role[k]], bufs[p->dst_role[k]]);
-      pfk_event_record(ev[k + 1]);
+      if (!err) err = pfk_event_record(ev[it * p->npass + k + 1]);
     }
+  for (int k = 0; k < p->npass; k++) ms[k] = 0;
+  for (int it = 0; it < iters && !err; it++)
     for (int k = 0; k < p->npass && !err; k++) {
       float t = 0;
-      if (pfk_event_elapsed_ms(ev[k], ev[k + 1], &t)) err = 1;
+      const int e0 = it * p->npass + k;
+      if (pfk_event_elapsed_ms(ev[e0], ev[e0 + 1], &t)) err = 1;
       ms[k] += t;
     }
-  }
   for (int k = 0; k < p->npass; k++) {
     ms[k] /= iters;
     bytes[k] = pa_pass_bytes(&p->pass[k]);
   }
-  for (int k = 0; k <= p->npass; k++) pfk_event_destroy(ev[k]);
+  for (int i = 0; i < made; i++)
+    if (ev[i]) pfk_event_destroy(ev[i]);
+  free(ev);
   return err;
 }
 
@@ -1092,7 +1103,8 @@ static const char *kernel_name(const pfk_pass *ps) {
   if (ps->n1) return "long-two-step";
   if (ps->r2r_path) return "r2r-inner-fft";
   const int fam = pfk_pass_kernels(ps);
-  const int lm = (ps->variant & 8) && ps->kind != PFK_R2R && ps->prec == PFK_F32;
+  const int lm = (ps->variant & 8) && ps->kind != PFK_R2R &&
+                 (ps->prec == PFK_F32 || pfk_pass_plain(ps));
   if (fam & PFK_HAS_MFMA) {
     static const char *nm[2][2] = {{"mfma-f32", "mfma-f32-lm"}, {"mfma-bf16x2", "mfma-bf16x2-lm"}};
     return nm[ps->mfma == 2][lm && ps->nf <= 1024];

@@ -218,15 +218,15 @@ int pa_tune_measure(pfk_pass *ps, const void *src, void *dst, unsigned effort) {
   const int reps = (effort & PFFT_EXHAUSTIVE) ? 10 : (effort & PFFT_PATIENT) ? 6 : 3;
   static const int cand[14][2] = {{0, 0}, {0, 1}, {2, 0},  {2, 1},  {4, 0},  {4, 1},  {6, 0},
                                   {6, 1}, {8, 0}, {8, 1}, {12, 0}, {12, 1}, {3, 0}, {1, 0}};
-  /* line-major lane order (complex64): c2c passes whose lines are
-   * contiguous on one side and contiguous or adjacent along the tile on the
-   * other (the kernels stage a tile-adjacent side through LDS), and real
-   * passes whose complex side is contiguous too */
-  const int lm_ok =
-      ps->prec == PFK_F32 &&
-      ((ps->kind == PFK_C2C && (ps->sin == 1 || ps->sout == 1) && (ps->sin == 1 || ps->tsin == 1) &&
-        (ps->sout == 1 || ps->tsout == 1)) ||
-       (ps->kind == PFK_R2C && ps->sout == 1) || (ps->kind == PFK_C2R && ps->sin == 1));
+  /* line-major lane order: c2c passes whose lines are contiguous on one
+   * side and contiguous or adjacent along the tile on the other (the
+   * kernels switch lane order at an LDS exchange), and real passes whose
+   * complex side is contiguous too; complex128 on plain passes only */
+  const int lm_shape =
+      (ps->kind == PFK_C2C && (ps->sin == 1 || ps->sout == 1) && (ps->sin == 1 || ps->tsin == 1) &&
+       (ps->sout == 1 || ps->tsout == 1)) ||
+      (ps->kind == PFK_R2C && ps->sout == 1) || (ps->kind == PFK_C2R && ps->sin == 1);
+  const int lm_ok = lm_shape && (ps->prec == PFK_F32 || pfk_pass_plain(ps));
   const int fam = pfk_pass_kernels(ps);
   void *e0 = NULL, *e1 = NULL;
   if (pfk_event_create(&e0) || pfk_event_create(&e1)) {

@@ -81,7 +81,9 @@ def _expected(line, prec, variant, remap):
         return "small"
     if pow2:
         name = {0: "fast", 2: "fast-narrow", 3: "fast", 4: "fast-grid", 6: "fast-narrow-grid"}
-        lm = (variant & 8) and kind != "r2r" and prec == "f32"
+        full = {"c2c": (nf, nf), "r2c": (n, nf + 1), "c2r": (nf + 1, n)}.get(kind)
+        plain = full is not None and "nin=%d@0 nout=%d@0" % full in line
+        lm = (variant & 8) and kind != "r2r" and (prec == "f32" or plain)
         return name[variant & 7] + ("-lm" if lm else "") + ("+xcd" if remap else "")
     return "small" if small else "general"
 
@@ -118,8 +120,11 @@ def test_measure_fills_cache_and_reuses(tune_all):
     _plan_lines("c2c", [64, 128, 256], TO, "f64")
     n1, t1 = P.wisdom_count(), P.tune_timings()
     # three passes; candidates per pass: eight power-of-two variants, the
-    # short-length kernel where the length has one (64 and 128), the general one
-    assert n1 == 3 and t1 - t0 == 10 + 10 + 9
+    # short-length kernel where the length has one (64 and 128), the general
+    # one, and four line-major variants on the passes whose lines are
+    # contiguous on a side (the 256-point line pass and the transposing
+    # 64-point pass; not the 128-point column pass)
+    assert n1 == 3 and t1 - t0 == (10 + 4) + 10 + (9 + 4)
     _plan_lines("c2c", [64, 128, 256], TO, "f64")
     assert P.wisdom_count() == n1 and P.tune_timings() == t1
     # different geometry -> new selections
