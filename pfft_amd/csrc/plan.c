@@ -39,6 +39,7 @@ typedef struct arrgeo {
   INT phys_last;          /* physical extent of dim rnk-1 (padding), 0 if = ext */
   INT h;                  /* interleaved tuple (howmany), innermost */
   INT opad;               /* elements added to every stride beyond the innermost dim */
+  int opad_outer;         /* 1: opad on the strides beyond the row stride only */
   int is_real;
 } arrgeo;
 
@@ -91,7 +92,7 @@ static int64_t geo_strides(const arrgeo *a, INT *stride) {
     const int d = a->order[k];
     stride[d] = run;
     run *= (d == a->rnk - 1 && a->phys_last) ? a->phys_last : a->ext[d];
-    if (k > 0) run += a->opad;
+    if (k > 0 && !(a->opad_outer && k == a->rnk - 1)) run += a->opad;
   }
   return (int64_t)run;
 }
@@ -472,6 +473,8 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
   const int pad_rows = ev ? (ev[0] == '1') : 1;
   ev = getenv("PFFT_AMD_PAD_PLANES");
   const int pad_planes = ev ? (ev[0] == '1') : 1;
+  ev = getenv("PFFT_AMD_ROW_OPAD");
+  const int row_opad = ev ? (ev[0] == '1') : -1;   /* -1: by the row's segment count */
   ev = getenv("PFFT_AMD_XCD_REMAP");
   const int xcd_remap = ev ? (ev[0] == '1') : 0;
   ev = getenv("PFFT_AMD_INTER_ORDER");
@@ -517,11 +520,19 @@ pa_plan *pa_plan_create(int prec, int rnk_n, const INT *n, const INT *ni, const 
       const INT align = (prec == PFK_F64 ? 8 : 16) * (is_r2r ? 2 : 1);
       if (pad_rows && howmany == 1 && nxt.ext[last] >= align && nxt.ext[last] % align)
         nxt.phys_last = (nxt.ext[last] + align - 1) / align * align;
-      /* every row and plane pitch one 128-byte segment longer than its
-       * contents, so that no stride of a strided pass is a multiple of a
-       * large power of two: lines of a tile then spread over the memory
-       * channels (a 4 MiB plane stride moved ~20% slower than a padded one) */
-      if (pad_rows && pad_planes && howmany == 1 && nxt.ext[last] >= align) nxt.opad = align;
+      /* every plane pitch one 128-byte segment longer than its contents,
+       * and every row pitch an odd number of segments (one more where the
+       * rounded row has an even count), so that no stride of a strided
+       * pass is a multiple of a large power of two: lines of a tile then
+       * spread over the memory channels (a 4 MiB plane stride moved ~20%
+       * slower than a padded one).  A row already odd keeps its length:
+       * 512^3 r2c rows of 264 complex128 (33 segments) padded to 272 made
+       * the column passes ~4% slower than at 264. */
+      if (pad_rows && pad_planes && howmany == 1 && nxt.ext[last] >= align) {
+        const INT row = nxt.phys_last ? nxt.phys_last : nxt.ext[last];
+        nxt.opad = align;
+        nxt.opad_outer = row_opad >= 0 ? !row_opad : (row % align == 0 && (row / align) % 2 == 1);
+      }
     }
     const int skipped = p->skip[d < r ? d : r] != 0;
     int kind = PFK_C2C;
