@@ -217,6 +217,26 @@ __device__ __forceinline__ int lidx(int p, int c) {
 template <int NF, int C, bool LM>
 constexpr int lds_slots() { return LM ? C * lm_pitch<NF>() : NF * C; }
 
+// In line-major order a line's NF/E lanes are consecutive threads, so with
+// at most 64 of them every line lies inside one wavefront, and an exchange
+// within lines only reorders data among the lanes of one wave.  A wave's own
+// LDS accesses execute in order, so such an exchange needs only the
+// compiler to keep them in order (wavefront-scope fences), not a workgroup
+// barrier that would hold every wave of the tile.
+template <int NF, int E, bool LM>
+constexpr bool wave_lines() { return LM && NF / E <= 64; }
+
+template <bool WAVE>
+__device__ __forceinline__ void line_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+  } else {
+    lds_barrier();
+  }
+}
+
 // Exchange through LDS: write v[m] at position pos[m], read back t + T*m.
 template <typename T, int NF, int C, int E, bool LM>
 __device__ __forceinline__ void exchange(C2<T> *v, const int *pos, C2<T> *lds, const int t,
@@ -224,10 +244,10 @@ __device__ __forceinline__ void exchange(C2<T> *v, const int *pos, C2<T> *lds, c
   constexpr int TT = NF / E;
 #pragma unroll
   for (int m = 0; m < E; ++m) lds[lidx<NF, C, LM>(pos[m], c)] = v[m];
-  lds_barrier();
+  line_sync<wave_lines<NF, E, LM>()>();
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = lds[lidx<NF, C, LM>(t + TT * m, c)];
-  lds_barrier();
+  line_sync<wave_lines<NF, E, LM>()>();
 }
 
 // Full length-NF FFT on registers; first stage input = v (positions t+T*m),
@@ -587,7 +607,9 @@ k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       }
     }
     if (!more) return false;
-    lds_barrier();   // the next tile reuses the LDS exchange buffer
+    // the next tile reuses the LDS exchange buffer (staged lane orders
+    // cross waves)
+    line_sync<XS == 0 && wave_lines<NF, E, LM>()>();
     item = ni;
     tc = tn;
     return true;
@@ -659,7 +681,7 @@ k_r2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
     // mirror exchange: Z_{h-k}
 #pragma unroll
     for (int m = 0; m < E; ++m) lds[lidx<NF, C, LM>(t + TT * m, c)] = cur[m];
-    lds_barrier();
+    line_sync<wave_lines<NF, E, LM>()>();
     if (PL && valid) {
       // every output k = t + T*m (and the Nyquist bin, thread t = 0) is kept
       C2<T> *d = reinterpret_cast<C2<T> *>(dst) + tc.db + (int64_t)c * P.tsout;
@@ -704,7 +726,7 @@ k_r2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       }
     }
     if (!more) return false;
-    lds_barrier();
+    line_sync<wave_lines<NF, E, LM>()>();
     item = ni;
     tc = tn;
     return true;
@@ -797,7 +819,7 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
         lds[lidx<NF, C, LM>(H, c)] = okh ? C2<T>{curh.x * msign<T>(P.imod, jh), (T)0} : C2<T>{0, 0};
       }
     }
-    lds_barrier();
+    line_sync<wave_lines<NF, E, LM>()>();
     // Z_k = E_k + i O_k
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -815,7 +837,7 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       }
       cur[m] = z;
     }
-    lds_barrier();
+    line_sync<wave_lines<NF, E, LM>()>();
     fft_regs<T, NF, C, E, SIGN, LM>(cur, lds, t, c, tw);
     if (valid) {
       C2<T> *d = reinterpret_cast<C2<T> *>(dst + tc.db + (int64_t)c * P.tsout);
@@ -831,7 +853,7 @@ k_c2r_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const 
       }
     }
     if (!more) return false;
-    lds_barrier();
+    line_sync<wave_lines<NF, E, LM>()>();
     item = ni;
     tc = tn;
     return true;
