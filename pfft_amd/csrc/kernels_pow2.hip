@@ -286,23 +286,26 @@ __host__ __device__ constexpr int n_exchanges(int nf, int e) {
 
 // Exchange through the line-major LDS layout with a change of lane order:
 // v[m] is written at position pos[m] of line cw and read back as position
-// tr + T*m of line cr.
-template <typename T, int NF, int C, int E>
+// tr + T*m of line cr.  WAVE: both sides in line-major order with every line
+// inside one wave (no change of order), so wavefront-scope ordering suffices.
+template <typename T, int NF, int C, int E, bool WAVE = false>
 __device__ __forceinline__ void exchange_x(C2<T> *v, const int *pos, C2<T> *lds, const int cw,
                                            const int tr, const int cr) {
   constexpr int TT = NF / E;
 #pragma unroll
   for (int m = 0; m < E; ++m) lds[lidx<NF, C, true>(pos[m], cw)] = v[m];
-  lds_barrier();
+  line_sync<WAVE>();
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = lds[lidx<NF, C, true>(tr + TT * m, cr)];
-  lds_barrier();
+  line_sync<WAVE>();
 }
 
 // The FFT of fft_regs with a change of lane order at exchange SW (1-based):
 // stages before it run as thread (ta, ca) (position index, line), the
 // exchange writes in that order and reads in order (tb, cb), and the later
-// stages run as (tb, cb).  K counts the exchanges done.
+// stages run as (tb, cb).  K counts the exchanges done.  One of the two
+// orders is line-major (the kernels stage tile-major loads or stores), so
+// every exchange but the switching one stays within line-major lines.
 template <typename T, int NF, int C, int E, int SIGN, int L, int K, int SW>
 __device__ __forceinline__ void fft_stages_sw(C2<T> *v, int *pos, C2<T> *lds, const int ta,
                                               const int ca, const int tb, const int cb,
@@ -316,7 +319,10 @@ __device__ __forceinline__ void fft_stages_sw(C2<T> *v, int *pos, C2<T> *lds, co
     stage<T, NF, E, E, L, SIGN, LAST>(v, pos, t, tw);
     if constexpr (!LAST) {
       constexpr bool AFTER = K + 1 >= SW;   // the next stage runs as (tb, cb)
-      exchange_x<T, NF, C, E>(v, pos, lds, c, AFTER ? tb : ta, AFTER ? cb : ca);
+      constexpr bool SWITCH = K + 1 == SW;
+      exchange_x<T, NF, C, E, !SWITCH && wave_lines<NF, E, true>()>(v, pos, lds, c,
+                                                                     AFTER ? tb : ta,
+                                                                     AFTER ? cb : ca);
       fft_stages_sw<T, NF, C, E, SIGN, L * E, K + 1, SW>(v, pos, lds, ta, ca, tb, cb, tw);
     }
   } else if constexpr (REM > 1) {
