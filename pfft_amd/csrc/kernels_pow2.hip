@@ -491,6 +491,13 @@ __device__ __forceinline__ void store_c2c_pair(const pfk_pass &P, float *dst, co
   }
 }
 
+// Waves per SIMD the compiler must leave room for: two 1024-thread
+// workgroups of the 512-point complex64 kernel per CU (at most 64 VGPRs; a
+// few are spilled), which measured 1-3% faster per pass than one
+// workgroup at 70 VGPRs; no constraint otherwise.
+template <typename T, int NF, int C, int E>
+constexpr int c2c_waves_per_eu() { return sizeof(T) == 4 && NF == 512 && C * NF / E == 1024 ? 8 : 1; }
+
 // LM: line-major lane order (thread = c*T + t: a wave's lanes run along a
 // line), for passes whose lines are contiguous in memory; otherwise
 // thread = t*C + c (a wave's lanes run across the tile's lines).
@@ -509,13 +516,6 @@ __device__ __forceinline__ void store_c2c_pair(const pfk_pass &P, float *dst, co
 // modulation): no masks or selects, and element addresses advance by a
 // uniform stride, which halves the vector instructions spent per element
 // outside the butterflies.
-// Waves per SIMD the compiler must leave room for: two 1024-thread
-// workgroups of the 512-point complex64 kernel per CU (at most 64 VGPRs; a
-// few are spilled), which measured 1-3% faster per pass than one
-// workgroup at 70 VGPRs; no constraint otherwise.
-template <typename T, int NF, int C, int E>
-constexpr int c2c_waves_per_eu() { return sizeof(T) == 4 && NF == 512 && C * NF / E == 1024 ? 8 : 1; }
-
 template <typename T, int NF, int C, int E, int SIGN, bool LM = false, bool TW = false,
           bool PL = false, int XS = 0, int PAIR = 0>
 __global__ void __launch_bounds__(C *NF / E)
