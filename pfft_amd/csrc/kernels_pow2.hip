@@ -491,13 +491,6 @@ __device__ __forceinline__ void store_c2c_pair(const pfk_pass &P, float *dst, co
   }
 }
 
-// Waves per SIMD the compiler must leave room for: two 1024-thread
-// workgroups of the 512-point complex64 kernel per CU (at most 64 VGPRs; a
-// few are spilled), which measured 1-3% faster per pass than one
-// workgroup at 70 VGPRs; no constraint otherwise.
-template <typename T, int NF, int C, int E>
-constexpr int c2c_waves_per_eu() { return sizeof(T) == 4 && NF == 512 && C * NF / E == 1024 ? 8 : 1; }
-
 // LM: line-major lane order (thread = c*T + t: a wave's lanes run along a
 // line), for passes whose lines are contiguous in memory; otherwise
 // thread = t*C + c (a wave's lanes run across the tile's lines).
@@ -519,7 +512,6 @@ constexpr int c2c_waves_per_eu() { return sizeof(T) == 4 && NF == 512 && C * NF 
 template <typename T, int NF, int C, int E, int SIGN, bool LM = false, bool TW = false,
           bool PL = false, int XS = 0, int PAIR = 0>
 __global__ void __launch_bounds__(C *NF / E)
-__attribute__((amdgpu_waves_per_eu(c2c_waves_per_eu<T, NF, C, E>())))
 k_c2c_pow2(const pfk_pass P, const int64_t item0, const int64_t item_end, const int64_t ntiles,
            const T *__restrict__ src, T *__restrict__ dst) {
   constexpr int TT = NF / E;
@@ -1048,7 +1040,10 @@ static int launch_nf(const pfk_pass *p, const void *src, void *dst) {
 // C = 8 for 16-byte FP64 complex, C = 16 for 8-byte FP32 complex (C = 4 / 8
 // at the longest lengths, where the tile would exceed 128 KiB of LDS).
 // E = 16 elements per thread from NF = 1024 on keeps the workgroup at
-// <= 1024 threads.
+// <= 1024 threads; complex64 takes E = 16 from NF = 512 (512 threads in
+// two radix-16 stages and a radix-2 one, 95-117 VGPRs without spills, two
+// workgroups per CU: its column passes of the 512^3 pair 207-212 -> 200 us
+// against E = 8 at 1024 threads).
 static int launch_pow2_f64(const pfk_pass *p, const void *src, void *dst) {
   switch (p->nf) {
     case 64: return launch_nf<double, 64, 8, 8>(p, src, dst);
@@ -1081,7 +1076,7 @@ static int launch_pow2_f32(const pfk_pass *p, const void *src, void *dst) {
     case 64: return launch_nf<float, 64, 16, 8>(p, src, dst);
     case 128: return launch_nf<float, 128, 16, 8>(p, src, dst);
     case 256: return launch_nf<float, 256, 16, 8>(p, src, dst);
-    case 512: return launch_nf<float, 512, 16, 8>(p, src, dst);
+    case 512: return launch_nf<float, 512, 16, 16>(p, src, dst);
     case 1024: return launch_nf<float, 1024, 16, 16>(p, src, dst);
     case 2048: return launch_nf<float, 2048, 8, 16>(p, src, dst);
     case 4096: return launch_nf<float, 4096, 4, 16>(p, src, dst);
